@@ -1,0 +1,222 @@
+/*
+ * unap_hip.h -- C-ABI of the MI355X (gfx950) LDU solve path.
+ *
+ * Plain C types only (no HIP or torch types): opaque handles, int64 labels
+ * (the reference's `label` under -DLABEL_INT64, SConstruct:52) and double
+ * scalars.  Every call returns an int status (UNAP_HIP_OK = 0, negative on
+ * error, message in unap_hip_last_error()) and never exits the process,
+ * unlike the reference's CHECK_POINTER/ERROR_EXIT (src/base/unap.hpp:96-101).
+ *
+ * Vectors are in the caller's cell numbering.  `where` = UNAP_HIP_HOST for
+ * host pointers, UNAP_HIP_DEVICE for device pointers (HBM-resident data, no
+ * PCIe copy).  create() calls deep-copy their inputs, as the reference's
+ * non-reUse constructors do (lduMatrix.cpp:28-68).
+ *
+ * Threading: one process-wide HIP stream; calls are not re-entrant (the
+ * reference is not either).  One GPU (the current HIP device).
+ */
+#ifndef UNAP_HIP_H
+#define UNAP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UNAP_HIP_OK 0
+#define UNAP_HIP_ERR_ARG (-1)        /* bad argument */
+#define UNAP_HIP_ERR_HIP (-2)        /* HIP runtime error (no device, OOM, ...) */
+#define UNAP_HIP_ERR_FACE_ORDER (-3) /* sweep needs owner-sorted faces, owner < neighbour */
+#define UNAP_HIP_ERR_STATE (-4)      /* handle used in a state that does not allow it */
+#define UNAP_HIP_ERR_INTERNAL (-5)
+
+#define UNAP_HIP_HOST 0
+#define UNAP_HIP_DEVICE 1
+
+typedef struct unap_hip_ldu_s* unap_hip_ldu;
+typedef struct unap_hip_precond_s* unap_hip_precond;
+typedef struct unap_hip_smoother_s* unap_hip_smoother;
+typedef struct unap_hip_aggl_s* unap_hip_aggl;
+typedef struct unap_hip_solver_s* unap_hip_solver;
+
+/* Last error message of this thread ("" if none). */
+const char* unap_hip_last_error(void);
+
+/* Order of every FP64 dot product and norm (no reference counterpart; a parity
+   aid).  UNAP_HIP_REDUCE_TREE (default): wave64 shuffle trees and fixed-order
+   block partials, deterministic run to run.  UNAP_HIP_REDUCE_REFERENCE: one
+   thread sums in the caller's cell order, exactly the serial loops of
+   unapVectorOperations.cpp:21-30 and unapVector.hpp:478-489, so FP64 solves
+   reproduce the reference bit for bit (slow: O(n) sequential per reduction). */
+#define UNAP_HIP_REDUCE_TREE 0
+#define UNAP_HIP_REDUCE_REFERENCE 1
+int unap_hip_set_reduction_order(int order);
+
+/* Device name and compute-unit count of the current device. */
+int unap_hip_device_info(char* name, int64_t name_len, int64_t* compute_units);
+
+/* ------------------------------------------------------------- LDU matrix
+ * Replaces lduMatrix(nCells, lowerAddr, upperAddr, lower, diag, upper)
+ * (src/matrix/lduMatrix/lduMatrix.hpp:80-88, lduMatrix.cpp:28-68).  As there,
+ * lower == upper (the same pointer) or lower == NULL makes the matrix
+ * symmetric (one coefficient array).  Face f couples lowerAddr[f] (owner,
+ * coefficient upper[f] in its row) and upperAddr[f] (neighbour, lower[f]).
+ * Amul accepts any face order; the sweeps (DIC, DILU, Gauss-Seidel) need faces
+ * sorted by owner with owner < neighbour, as the reference's do. */
+int unap_hip_ldu_create(int64_t n_cells, int64_t n_faces, const int64_t* lower_addr,
+                        const int64_t* upper_addr, const double* lower, const double* diag,
+                        const double* upper, unap_hip_ldu* out);
+int unap_hip_ldu_destroy(unap_hip_ldu A);
+/* size() / nFaces / symm() (lduMatrix.hpp:210-231) and the level-schedule depth
+   of the forward and backward sweeps (0 when faces are not owner-sorted). */
+int unap_hip_ldu_info(unap_hip_ldu A, int64_t* n_cells, int64_t* n_faces, int* symmetric,
+                      int64_t* fwd_levels, int64_t* bwd_levels);
+/* setMatrixCoeffients(diag, upper, lower) (lduMatrix.cpp:508-529): new values,
+   same addressing; lower == upper or NULL keeps one coefficient array. */
+int unap_hip_ldu_set_coeffs(unap_hip_ldu A, const double* diag, const double* upper,
+                            const double* lower);
+/* spMV(Apsi, psi) (src/matrix/lduMatrix/lduMatrixSpMV.cpp:16-98). */
+int unap_hip_ldu_spmv(unap_hip_ldu A, const double* psi, double* Apsi, int where);
+
+/* --------------------------------------------------------- preconditioners
+ * Replaces matrix::preconditioner::precondition(w, r) (unapMatrix.hpp:337)
+ * and its ldu implementations; type codes 1/2/3 are the reference's Fortran
+ * precond codes (fortranInterface.cpp:227-238). */
+#define UNAP_HIP_PRECOND_NONE 0 /* identity, unapMatrixPrecond.cpp:4-17 */
+#define UNAP_HIP_PRECOND_DIAG 1 /* lduDiagPrecond.cpp:13-73 */
+#define UNAP_HIP_PRECOND_DIC 2  /* lduDICPrecond.cpp:4-77 */
+#define UNAP_HIP_PRECOND_DILU 3 /* lduDILUPrecond.cpp:4-86 */
+#define UNAP_HIP_PRECOND_AMG 4  /* MGPreconditioner.hpp:11-49 (a stub there; see below) */
+
+#define UNAP_HIP_SMOOTHER_GS 1    /* lduGaussSeidelSmoother.cpp:3-80 */
+#define UNAP_HIP_SMOOTHER_CHEBY 2 /* chebySmoother.cpp:13-149 */
+
+/* Agglomeration-AMG preconditioner: precondition(w, r) = { w = 0;
+   MGSolver::solve(w, A, r) } running exactly n_vcycles V-cycles
+   (MG.hpp:8-121, Vcycle.cpp:27-332) over the pairwise agglomeration of the
+   face weights (|upper| when weights == NULL, MGPreconditioner.cpp:14-19).
+   precision 64: the reference's arithmetic in FP64 (coarsest level solved by
+   PCG+DIC / PBiCGStab+DILU to relTol 1e-2, solveCoarsestLevel.cpp:10-55).
+   precision 32: FP32 hierarchy, smoothers and transfers, Chebyshev estimates
+   taken once at setup, coarsest level solved directly in FP64 (a dense
+   Cholesky inverse); use it under the flexible solver UNAP_HIP_SOLVER_FCG. */
+typedef struct {
+    int64_t smoother;               /* UNAP_HIP_SMOOTHER_* (default GS) */
+    int64_t n_cells_coarsest;       /* SET_nCellsInCoarsestLevel (default 50) */
+    int64_t max_levels;             /* SET_maxLevels (default 50) */
+    int64_t n_vcycles;              /* V-cycles per application (default 2, MGPreconditioner.cpp:11) */
+    int64_t n_pre_sweeps;           /* MG.cpp:11-13 defaults 0 / 2 / 2 */
+    int64_t n_post_sweeps;
+    int64_t n_finest_sweeps;
+    int64_t precision;              /* 64 or 32 */
+} unap_hip_amg_params;
+
+int unap_hip_amg_params_default(unap_hip_amg_params* p);
+int unap_hip_precond_create(unap_hip_ldu A, int type, const unap_hip_amg_params* amg,
+                            const double* face_weights, unap_hip_precond* out);
+int unap_hip_precond_destroy(unap_hip_precond P);
+int unap_hip_precond_apply(unap_hip_precond P, const double* r, double* w, int where);
+/* rD() (unapMatrix.hpp:343): reciprocal (preconditioned) diagonal, host copy. */
+int unap_hip_precond_rd(unap_hip_precond P, double* rD);
+
+/* ----------------------------------------------------------------- smoothers
+ * Replaces matrix::smoother::smooth(x, A, b, nSweeps) and init()
+ * (unapMatrix.hpp:351-373).  The Chebyshev smoother keeps the reference's
+ * behaviour: its first call after creation or init() only runs the
+ * eigenvalue estimate (eigenDiagPCG, 10 Jacobi-PCG steps that update x). */
+int unap_hip_smoother_create(int type, unap_hip_smoother* out);
+int unap_hip_smoother_destroy(unap_hip_smoother S);
+int unap_hip_smoother_smooth(unap_hip_smoother S, unap_hip_ldu A, double* x, const double* b,
+                             int64_t n_sweeps, int where);
+int unap_hip_smoother_init(unap_hip_smoother S);
+/* eigenDiagPCG (eigenDiagPCG.cpp:11-40): n_pcgs Jacobi-PCG steps on (A, x, b)
+   updating x, then the Lanczos/Sturm estimate of the largest eigenvalue. */
+int unap_hip_eigen_estimate(unap_hip_ldu A, double* x, const double* b, int64_t n_pcgs,
+                            int where, double* lambda_max);
+
+/* ------------------------------------------------------------- agglomeration
+ * Replaces lduAgglomeration (lduAgglomeration.hpp:8-97): agglomerate(weights)
+ * with SET_maxLevels / SET_nCellsInCoarsestLevel, the per-level maps and the
+ * Galerkin coarse matrices (agglomerateMatrix).  Built on the host. */
+int unap_hip_aggl_create(unap_hip_ldu A, const double* weights, int64_t max_levels,
+                         int64_t n_cells_coarsest, unap_hip_aggl* out);
+int unap_hip_aggl_destroy(unap_hip_aggl G);
+int unap_hip_aggl_nlevels(unap_hip_aggl G, int64_t* n_levels);
+int unap_hip_aggl_level_size(unap_hip_aggl G, int64_t level, int64_t* n_cells, int64_t* n_faces);
+/* restrictAddressing / faceRestrictAddressing (lduAgglomeration.hpp:83-92) */
+int unap_hip_aggl_restrict_addressing(unap_hip_aggl G, int64_t level, int64_t* out);
+int unap_hip_aggl_face_restrict_addressing(unap_hip_aggl G, int64_t level, int64_t* out);
+/* coarseMatrixLevels(level) after agglomerateMatrix (lduAgglomerateMatrix.cpp:3-104) */
+int unap_hip_aggl_coarse_matrix(unap_hip_aggl G, int64_t level, int64_t* lower_addr,
+                                int64_t* upper_addr, double* lower, double* diag, double* upper,
+                                int* symmetric);
+/* restrictField / prolongField (unapMatrix.hpp:378-438) on the GPU; level is the
+   fine level index of the restriction / the coarse level index of the prolongation. */
+int unap_hip_aggl_restrict_field(unap_hip_aggl G, int64_t level, const double* fine, double* coarse,
+                                 int where);
+int unap_hip_aggl_prolong_field(unap_hip_aggl G, int64_t level, const double* coarse, double* fine,
+                                int where);
+
+/* ------------------------------------------------------------------ solvers
+ * Replaces matrix::solver (unapMatrix.hpp:247-323): PCG (PCG.cpp:35-183),
+ * PBiCGStab (PBiCGStab.cpp:45-459), MGSolver (MGSolve.cpp:17-187) and a
+ * flexible CG (no reference counterpart; the outer loop of the
+ * mixed-precision path).  Defaults as solver::solver() (unapMatrixSolver.cpp:3-10):
+ * maxIter 1000, minIter 0, relTol 1e-6, tol 0. */
+#define UNAP_HIP_SOLVER_PCG 1
+#define UNAP_HIP_SOLVER_PBICGSTAB 2
+#define UNAP_HIP_SOLVER_MG 3
+#define UNAP_HIP_SOLVER_FCG 4
+
+/* solverPerformance (unapMatrix.hpp:43-174) */
+typedef struct {
+    double initial_residual;
+    double final_residual;
+    double previous_residual;
+    int64_t n_iterations;
+    int32_t converged;
+    int32_t singular;
+} unap_hip_perf;
+
+int unap_hip_solver_create(int type, unap_hip_solver* out);
+int unap_hip_solver_destroy(unap_hip_solver S);
+/* PCG(precond) / PBiCGStab(precond) / FCG(precond); NULL = identity */
+int unap_hip_solver_set_precond(unap_hip_solver S, unap_hip_precond P);
+/* SET_maxIter / SET_minIter / SET_relTol / SET_tolerance */
+int unap_hip_solver_set_limits(unap_hip_solver S, int64_t max_iter, int64_t min_iter,
+                               double rel_tol, double tol);
+/* MGSolver(A, agglomeration, smoothers) with SET_nPreSweeps/nPostSweeps/
+   nFinestSweeps; weights NULL -> |upper| / ||upper||_2 (fortranInterface.cpp:383-395). */
+int unap_hip_solver_set_mg(unap_hip_solver S, const unap_hip_amg_params* amg, const double* weights);
+/* solve(x, A, b): x is the initial guess and the result. */
+int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const double* b, int where,
+                          unap_hip_perf* perf);
+/* Residual history of the last solve: initial residual, then one per iteration. */
+int unap_hip_solver_history(unap_hip_solver S, double* out, int64_t cap, int64_t* len);
+/* Device time of the last solve in milliseconds (HIP events around the solve). */
+int unap_hip_solver_last_ms(unap_hip_solver S, double* ms);
+
+/* ------------------------------------------------------------------ batches
+ * Many independent LDU systems solved in one persistent kernel launch: one
+ * workgroup runs one system's whole Krylov solve, systems are taken from a
+ * work queue largest-first.  System s has cells [cell_off[s], cell_off[s+1])
+ * and faces [face_off[s], face_off[s+1]) of the concatenated arrays, with
+ * addresses local to the system and faces owner-sorted.  solver[s] is
+ * UNAP_HIP_SOLVER_PCG (with DIC) or UNAP_HIP_SOLVER_PBICGSTAB (with DILU);
+ * lower may be NULL for symmetric systems (then every system uses upper).
+ * The preconditioners run in FP32, the Krylov loop in FP64. */
+typedef struct unap_hip_batch_s* unap_hip_batch;
+int unap_hip_batch_create(int64_t n_systems, const int64_t* cell_off, const int64_t* face_off,
+                          const int64_t* lower_addr, const int64_t* upper_addr, const double* lower,
+                          const double* diag, const double* upper, const int32_t* solver,
+                          unap_hip_batch* out);
+int unap_hip_batch_destroy(unap_hip_batch B);
+int unap_hip_batch_solve(unap_hip_batch B, double* x, const double* b, int where, int64_t max_iter,
+                         double rel_tol, double tol, unap_hip_perf* perf);
+int unap_hip_batch_last_ms(unap_hip_batch B, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,235 @@
+// coarse.hpp -- the coarsest AMG level solved inside one workgroup.
+//
+// MGSolver::solveCoarsestLevel (src/solvers/MG/solveCoarsestLevel.cpp:10-55)
+// runs PCG + DIC (symmetric) or PBiCGStab + DILU to relTol 1e-2, at most 100
+// iterations, from a zero guess.  The coarsest level has a few hundred to a
+// few thousand cells, so the whole solve is latency-bound: one 1024-thread
+// workgroup keeps every vector in LDS, walks the DIC/DILU level schedule with
+// workgroup barriers and reduces with the wave64 shuffle tree.  The arithmetic
+// per row is the reference's; reductions differ only in summation order.
+#pragma once
+
+#include "ops.hpp"
+
+namespace unap_hip {
+
+struct CoarseArgs {
+    DMat<double> A;   // Amul coefficients
+    DMat<double> AP;  // preconditioner coefficients (DIC: upper both sides; DILU: as A)
+    const double* rD;
+    const int* fwdStart;
+    int nFwd;
+    const int* bwdRows;
+    const int* bwdStart;
+    int nBwd;
+    const double* src;
+    double* corr;
+    double relTol, tol;
+    int maxIter;
+    const int* iperm;  // external -> internal (reference-order reductions)
+    int serial;
+};
+
+// Sum over the workgroup, broadcast to every thread (fixed order).
+template <int NS>
+__device__ __forceinline__ void wgSum(double (&v)[NS], double* red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double a = v[s];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
+        v[s] = a;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) red[s * 16 + wave] = v[s];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double a = 0.0;
+        for (int w = 0; w < nw; ++w) a += red[s * 16 + w];
+        v[s] = a;
+    }
+    __syncthreads();
+}
+
+// Sum term(i) over the level: tree over the workgroup, or (serial mode) one
+// thread walking the reference's cell order.  Every thread gets the totals.
+// The vectors term() reads must be complete (the caller has synchronised).
+template <int NS, class F>
+__device__ __forceinline__ void wgReduce(const CoarseArgs& a, int n, F term, double (&v)[NS], double* red) {
+    if (a.serial) {
+        if (threadIdx.x == 0) {
+            double acc[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+            for (int e = 0; e < n; ++e) {
+                double t[NS];
+                term(a.iperm[e], t);
+#pragma unroll
+                for (int s = 0; s < NS; ++s) acc[s] = acc[s] + t[s];
+            }
+#pragma unroll
+            for (int s = 0; s < NS; ++s) red[48 + s] = acc[s];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NS; ++s) v[s] = red[48 + s];
+        __syncthreads();
+        return;
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) v[s] = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double t[NS];
+        term(i, t);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) v[s] += t[s];
+    }
+    wgSum<NS>(v, red);
+}
+
+__device__ __forceinline__ void wgPrecond(const CoarseArgs& a, double* w, const double* r) {
+    for (int k = 0; k < a.nFwd; ++k) {
+        for (int q = a.fwdStart[k] + threadIdx.x; q < a.fwdStart[k + 1]; q += blockDim.x)
+            fwdRow<double, double>(a.AP, q, w, r, a.rD);
+        __syncthreads();
+    }
+    for (int k = 0; k < a.nBwd; ++k) {
+        for (int q = a.bwdStart[k] + threadIdx.x; q < a.bwdStart[k + 1]; q += blockDim.x)
+            bwdRow<double, double>(a.AP, a.bwdRows[q], w, a.rD);
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ bool wgConv(double fin, double init, double tol, double relTol, long long it) {
+    return (fin < tol || (relTol > kSmallD && fin <= relTol * init)) && it >= 0;
+}
+
+// PCG + DIC (PCG.cpp:35-183) on the coarsest level.
+__global__ void __launch_bounds__(1024) kCoarsePCG(CoarseArgs a, Skip sk) {
+    if (sk.on()) return;
+    extern __shared__ double lds[];
+    const int n = a.A.n;
+    double* x = lds;
+    double* r = x + n;
+    double* w = r + n;
+    double* p = w + n;
+    double* red = p + n;
+    const int t = threadIdx.x, nt = blockDim.x;
+    double s1[1];
+    // x = 0; A x = 0 exactly, so r = b - A x = b
+    for (int i = t; i < n; i += nt) {
+        x[i] = 0.0;
+        r[i] = a.src[i];
+        p[i] = 0.0;
+    }
+    __syncthreads();
+    wgReduce<1>(a, n, [&](int i, double(&q)[1]) { q[0] = r[i] * r[i]; }, s1, red);
+    const double init = sqrt(s1[0]) + kSmallD;
+    double rho = 1.0e37, rhoOld;
+    long long nIter = 0;
+    for (;;) {
+        rhoOld = rho;
+        wgPrecond(a, w, r);
+        wgReduce<1>(a, n, [&](int i, double(&q)[1]) { q[0] = w[i] * r[i]; }, s1, red);
+        rho = s1[0];
+        if (!(fabs(rho) > kSmallD)) break;
+        const double beta = rho / rhoOld;
+        for (int i = t; i < n; i += nt) p[i] = (nIter == 0) ? w[i] : w[i] + beta * p[i];
+        __syncthreads();
+        for (int i = t; i < n; i += nt) w[i] = rowAmul<double, double>(a.A, i, p);
+        __syncthreads();
+        wgReduce<1>(a, n, [&](int i, double(&q)[1]) { q[0] = w[i] * p[i]; }, s1, red);
+        const double alpha = rho / s1[0];
+        for (int i = t; i < n; i += nt) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * w[i];
+        }
+        __syncthreads();
+        wgReduce<1>(a, n, [&](int i, double(&q)[1]) { q[0] = r[i] * r[i]; }, s1, red);
+        const double fin = sqrt(s1[0]) + kSmallD;
+        ++nIter;
+        if (!(nIter < a.maxIter)) break;
+        if (wgConv(fin, init, a.tol, a.relTol, nIter)) break;
+    }
+    for (int i = t; i < n; i += nt) a.corr[i] = x[i];
+}
+
+// PBiCGStab + DILU (PBiCGStab.cpp:45-459) on the coarsest level.
+__global__ void __launch_bounds__(1024) kCoarseBiCG(CoarseArgs a, Skip sk) {
+    if (sk.on()) return;
+    extern __shared__ double lds[];
+    const int n = a.A.n;
+    double* x = lds;
+    double* r = x + n;
+    double* r0 = r + n;
+    double* p = r0 + n;
+    double* y = p + n;
+    double* Ay = y + n;
+    double* s = Ay + n;
+    double* z = s + n;
+    double* tt = z + n;
+    double* red = tt + n;
+    const int t = threadIdx.x, nt = blockDim.x;
+    double s1[1], s2[2];
+    for (int i = t; i < n; i += nt) {
+        x[i] = 0.0;
+        r[i] = a.src[i];
+        r0[i] = r[i];
+        p[i] = 0.0;
+        Ay[i] = 0.0;
+    }
+    __syncthreads();
+    wgReduce<1>(a, n, [&](int i, double(&q)[1]) { q[0] = r[i] * r[i]; }, s1, red);
+    const double init = sqrt(s1[0]) + kSmallD;
+    double rho = 0, alpha = 0, omega = 0;
+    long long nIter = 0;
+    for (;;) {
+        const double rhoOld = rho;
+        wgReduce<1>(a, n, [&](int i, double(&q)[1]) { q[0] = r0[i] * r[i]; }, s1, red);
+        rho = s1[0];
+        if (!(fabs(rho) > kSmallD)) break;
+        const double beta = (rho / rhoOld) * (alpha / omega);
+        for (int i = t; i < n; i += nt)
+            p[i] = (nIter == 0) ? r[i] : r[i] + beta * (p[i] - omega * Ay[i]);
+        __syncthreads();
+        wgPrecond(a, y, p);
+        for (int i = t; i < n; i += nt) Ay[i] = rowAmul<double, double>(a.A, i, y);
+        __syncthreads();
+        wgReduce<1>(a, n, [&](int i, double(&q)[1]) { q[0] = r0[i] * Ay[i]; }, s1, red);
+        alpha = rho / s1[0];
+        for (int i = t; i < n; i += nt) s[i] = r[i] - alpha * Ay[i];
+        __syncthreads();
+        wgReduce<1>(a, n, [&](int i, double(&q)[1]) { q[0] = s[i] * s[i]; }, s1, red);
+        double fin = sqrt(s1[0]) + kSmallD;
+        if (wgConv(fin, init, a.tol, a.relTol, nIter + 1)) {
+            for (int i = t; i < n; i += nt) x[i] += alpha * y[i];
+            __syncthreads();
+            break;
+        }
+        wgPrecond(a, z, s);
+        for (int i = t; i < n; i += nt) tt[i] = rowAmul<double, double>(a.A, i, z);
+        __syncthreads();
+        wgReduce<2>(a, n, [&](int i, double(&q)[2]) {
+            q[0] = tt[i] * tt[i];
+            q[1] = tt[i] * s[i];
+        }, s2, red);
+        omega = s2[1] / s2[0];
+        for (int i = t; i < n; i += nt) {
+            x[i] += alpha * y[i] + omega * z[i];
+            r[i] = s[i] - omega * tt[i];
+        }
+        __syncthreads();
+        wgReduce<1>(a, n, [&](int i, double(&q)[1]) { q[0] = r[i] * r[i]; }, s1, red);
+        fin = sqrt(s1[0]) + kSmallD;
+        ++nIter;
+        if (!(nIter < a.maxIter)) break;
+        if (wgConv(fin, init, a.tol, a.relTol, nIter)) break;
+    }
+    for (int i = t; i < n; i += nt) a.corr[i] = x[i];
+}
+
+}  // namespace unap_hip

@@ -1,0 +1,187 @@
+// device.hpp -- host-side plumbing of the HIP solve path: errors, device
+// buffers, the process-wide stream and reduction scratch, and the device copy
+// of one matrix (DevMatrix) in the layout planned by host_setup.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "host_setup.hpp"
+#include "kernels.hpp"
+
+namespace unap_hip {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define UH_CHECK(expr)                                                                       \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw ::unap_hip::Error(-2, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define UH_LAUNCHED() UH_CHECK(hipGetLastError())
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) UH_CHECK(hipMalloc((void**)&p, sizeof(T) * count));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        if (count > n) throw Error(-5, "upload larger than buffer");
+        if (count) UH_CHECK(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) {
+        if (v.size() != n) alloc(v.size());
+        upload(v.data(), v.size(), s);
+    }
+    T* get() const { return p; }
+};
+
+// Process-wide execution context: one stream, reduction partials.
+struct Ctx {
+    hipStream_t stream = nullptr;
+    int serialReductions = 0;  // 1: reductions in the reference's serial order
+    DBuf<double> partials;
+    DBuf<KState> scratchState;  // for standalone calls (eigen estimate, precond apply)
+    static Ctx& get();
+    void sync() { UH_CHECK(hipStreamSynchronize(stream)); }
+};
+
+struct Launch {
+    int grid, chunk;
+};
+inline Launch rowLaunch(int n) {
+    if (n <= 0) return {1, kBlock};
+    long long g = (n + kBlock - 1) / kBlock;
+    if (g > kMaxBlocks) g = kMaxBlocks;
+    long long chunk = (n + g - 1) / g;
+    chunk = (chunk + 63) / 64 * 64;
+    g = (n + chunk - 1) / chunk;
+    if (g >= 8) g = (g + 7) / 8 * 8;
+    if (g > kMaxBlocks) g = kMaxBlocks;
+    return {(int)g, (int)chunk};
+}
+
+template <class Body>
+void launchRows(const Body& b, int n, Skip sk = {nullptr, nullptr}) {
+    if (n <= 0) return;
+    Ctx& c = Ctx::get();
+    const Launch L = rowLaunch(n);
+    kRows<Body><<<L.grid, kBlock, 0, c.stream>>>(b, n, L.chunk, sk);
+    UH_LAUNCHED();
+}
+
+// Row kernel with NS sums followed by the one-block finalize.  `order` is the
+// level's external->internal map, walked by the reference-order mode.
+template <int NS, class Body, class Fin>
+void launchReduce(const Body& b, int n, const int* order, const Fin& fin, Skip sk = {nullptr, nullptr}) {
+    Ctx& c = Ctx::get();
+    const Launch L = rowLaunch(n);
+    kRowsReduce<Body, NS><<<L.grid, kBlock, 0, c.stream>>>(b, n, L.chunk, c.partials.get(), sk);
+    UH_LAUNCHED();
+    int nb = L.grid;
+    if (c.serialReductions) {
+        kSerialSum<Body, NS><<<1, 64, 0, c.stream>>>(b, order, n, c.partials.get(), sk);
+        UH_LAUNCHED();
+        nb = 1;
+    }
+    kFinalize<Fin, NS><<<1, kBlock, 0, c.stream>>>(fin, c.partials.get(), nb, sk);
+    UH_LAUNCHED();
+}
+
+// Device copy of one matrix in precision T.
+template <typename T>
+struct DevMatrix {
+    int n = 0;
+    DBuf<long long> sliceStart;
+    DBuf<uint32_t> rowInfo;
+    DBuf<int> col;
+    DBuf<T> val, diag;
+    DBuf<T> valU;  // upper on both sides (DIC on asymmetric storage), lazily
+    DBuf<int> perm, iperm;
+    DBuf<int> fwdStart, bwdRows, bwdStart;
+    std::vector<int> fwdStartH, bwdStartH;
+    bool sweepable = false;
+
+    DMat<T> view() const { return {n, sliceStart.get(), rowInfo.get(), col.get(), val.get(), diag.get()}; }
+    DMat<T> viewU() const {
+        return {n, sliceStart.get(), rowInfo.get(), col.get(), valU.p ? valU.get() : val.get(), diag.get()};
+    }
+
+    void build(const HostLdu& A, const Layout& L, hipStream_t s) {
+        n = L.n;
+        sweepable = L.sweepable;
+        sliceStart.upload(L.sliceStart, s);
+        rowInfo.upload(L.rowInfo, s);
+        col.upload(L.col, s);
+        std::vector<T> v;
+        fillSellValues<T>(L, A, false, v);
+        val.upload(v, s);
+        std::vector<T> d;
+        fillDiag<T>(L, A, d);
+        diag.upload(d, s);
+        perm.upload(L.perm, s);
+        iperm.upload(L.iperm, s);
+        fwdStartH = L.fwdStart;
+        bwdStartH = L.bwdStart;
+        fwdStart.upload(L.fwdStart, s);
+        bwdRows.upload(L.bwdRows, s);
+        bwdStart.upload(L.bwdStart, s);
+        UH_CHECK(hipStreamSynchronize(s));  // host staging vectors die here
+    }
+    void buildValU(const HostLdu& A, const Layout& L, hipStream_t s) {
+        if (A.symm || valU.p) return;
+        std::vector<T> v;
+        fillSellValues<T>(L, A, true, v);
+        valU.upload(v, s);
+        UH_CHECK(hipStreamSynchronize(s));
+    }
+    void updateValues(const HostLdu& A, const Layout& L, hipStream_t s) {
+        std::vector<T> v;
+        fillSellValues<T>(L, A, false, v);
+        val.upload(v, s);
+        std::vector<T> d;
+        fillDiag<T>(L, A, d);
+        diag.upload(d, s);
+        valU.release();
+        UH_CHECK(hipStreamSynchronize(s));
+    }
+};
+
+}  // namespace unap_hip

@@ -1,0 +1,377 @@
+// host_setup.cpp -- see host_setup.hpp.
+#include "host_setup.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+namespace unap_hip {
+
+static const double kSmall = 1.0e-37;  // SMALL, src/base/unap.hpp:88
+
+bool HostLdu::ownerSorted() const {
+    const int nf = nFaces();
+    for (int f = 0; f < nf; ++f) {
+        if (!(l[f] < u[f])) return false;
+        if (f > 0 && l[f] < l[f - 1]) return false;
+    }
+    return true;
+}
+
+Layout buildLayout(const HostLdu& A) {
+    Layout L;
+    const int n = A.n, nf = A.nFaces();
+    L.n = n;
+    L.nf = nf;
+    L.sweepable = A.ownerSorted();
+
+    // Per-row face lists in ascending face id (a single pass over faces keeps
+    // that order): side 0 = the row is the neighbour (coefficient lower[f],
+    // column owner), side 1 = the row is the owner (upper[f], column neighbour).
+    std::vector<int> rs(n + 1, 0);
+    for (int f = 0; f < nf; ++f) {
+        rs[A.u[f] + 1]++;
+        rs[A.l[f] + 1]++;
+    }
+    for (int c = 0; c < n; ++c) rs[c + 1] += rs[c];
+    std::vector<int> ent(2 * (size_t)nf), fillp(rs.begin(), rs.end() - 1);
+    std::vector<int> nLowerSide(n, 0);
+    for (int f = 0; f < nf; ++f) {
+        ent[fillp[A.u[f]]++] = 2 * f;  // neighbour side
+        ent[fillp[A.l[f]]++] = 2 * f + 1;
+        nLowerSide[A.u[f]]++;
+    }
+
+    // Forward level: 1 + max over lower-side neighbours (owners < row);
+    // backward level: 1 + max over upper-side neighbours.  Only meaningful
+    // for owner-sorted faces; otherwise every row is level 0.
+    std::vector<int> lev(n, 0), blev(n, 0);
+    int nlev = 1, nblev = 1;
+    if (L.sweepable) {
+        for (int c = 0; c < n; ++c) {
+            int m = 0;
+            for (int k = rs[c]; k < rs[c + 1]; ++k)
+                if ((ent[k] & 1) == 0) m = std::max(m, lev[A.l[ent[k] >> 1]] + 1);
+            lev[c] = m;
+            nlev = std::max(nlev, m + 1);
+        }
+        for (int c = n - 1; c >= 0; --c) {
+            int m = 0;
+            for (int k = rs[c]; k < rs[c + 1]; ++k)
+                if (ent[k] & 1) m = std::max(m, blev[A.u[ent[k] >> 1]] + 1);
+            blev[c] = m;
+            nblev = std::max(nblev, m + 1);
+        }
+    }
+
+    // Breadth-first order over the cell graph recovers spatial locality that
+    // the caller's numbering may have scrambled.
+    std::vector<int> bfs;
+    bfs.reserve(n);
+    {
+        std::vector<char> seen(n, 0);
+        for (int s = 0; s < n; ++s) {
+            if (seen[s]) continue;
+            seen[s] = 1;
+            size_t head = bfs.size();
+            bfs.push_back(s);
+            while (head < bfs.size()) {
+                const int c = bfs[head++];
+                for (int k = rs[c]; k < rs[c + 1]; ++k) {
+                    const int f = ent[k] >> 1;
+                    const int nb = (ent[k] & 1) ? A.u[f] : A.l[f];
+                    if (!seen[nb]) {
+                        seen[nb] = 1;
+                        bfs.push_back(nb);
+                    }
+                }
+            }
+        }
+    }
+
+    // Internal order: forward level major, breadth-first minor.
+    L.fwdStart.assign(nlev + 1, 0);
+    for (int c = 0; c < n; ++c) L.fwdStart[lev[c] + 1]++;
+    for (int k = 0; k < nlev; ++k) L.fwdStart[k + 1] += L.fwdStart[k];
+    {
+        std::vector<int> pos(L.fwdStart.begin(), L.fwdStart.end() - 1);
+        L.perm.assign(n, 0);
+        for (int i = 0; i < n; ++i) {
+            const int c = bfs[i];
+            L.perm[pos[lev[c]]++] = c;
+        }
+    }
+    L.iperm.assign(n, 0);
+    for (int i = 0; i < n; ++i) L.iperm[L.perm[i]] = i;
+
+    // Backward schedule: internal rows grouped by backward level, ascending.
+    L.bwdStart.assign(nblev + 1, 0);
+    for (int c = 0; c < n; ++c) L.bwdStart[blev[c] + 1]++;
+    for (int k = 0; k < nblev; ++k) L.bwdStart[k + 1] += L.bwdStart[k];
+    {
+        std::vector<int> pos(L.bwdStart.begin(), L.bwdStart.end() - 1);
+        L.bwdRows.assign(n, 0);
+        for (int i = 0; i < n; ++i) L.bwdRows[pos[blev[L.perm[i]]]++] = i;
+    }
+
+    // SELL-64: slice s holds internal rows 64s..64s+63, entry k of a row at
+    // sliceStart[s] + 64k + (row & 63).
+    L.nslices = (n + 63) / 64;
+    L.sliceStart.assign(L.nslices + 1, 0);
+    L.rowInfo.assign(n, 0);
+    for (int s = 0; s < L.nslices; ++s) {
+        int mx = 0;
+        for (int r = 64 * s; r < std::min(n, 64 * s + 64); ++r) {
+            const int c = L.perm[r];
+            const int len = rs[c + 1] - rs[c];
+            if (len > 0xffff) throw std::runtime_error("row longer than 65535 entries");
+            mx = std::max(mx, len);
+            L.rowInfo[r] = (uint32_t)len | ((uint32_t)nLowerSide[c] << 16);
+        }
+        L.maxRowLen = std::max(L.maxRowLen, mx);
+        L.sliceStart[s + 1] = L.sliceStart[s] + 64LL * mx;
+    }
+    L.nnzPadded = L.sliceStart[L.nslices];
+    L.col.assign(L.nnzPadded, 0);
+    L.src.assign(L.nnzPadded, -1);
+    for (int s = 0; s < L.nslices; ++s) {
+        const int mx = (int)((L.sliceStart[s + 1] - L.sliceStart[s]) / 64);
+        for (int lane = 0; lane < 64; ++lane) {
+            const int r = 64 * s + lane;
+            const int c = r < n ? L.perm[r] : -1;
+            const int len = r < n ? rs[c + 1] - rs[c] : 0;
+            for (int k = 0; k < mx; ++k) {
+                const long long p = L.sliceStart[s] + 64LL * k + lane;
+                if (k < len) {
+                    const int e = ent[rs[c] + k];
+                    const int f = e >> 1;
+                    L.col[p] = L.iperm[(e & 1) ? A.u[f] : A.l[f]];
+                    L.src[p] = e;
+                } else {
+                    L.col[p] = r < n ? r : 0;
+                    L.src[p] = -1;
+                }
+            }
+        }
+    }
+    return L;
+}
+
+template <typename T>
+void fillSellValues(const Layout& L, const HostLdu& A, bool useUpperBothSides, std::vector<T>& val) {
+    val.assign(L.nnzPadded, T(0));
+    const double* lw = useUpperBothSides ? A.up.data() : A.lower();
+    for (long long p = 0; p < L.nnzPadded; ++p) {
+        const int e = L.src[p];
+        if (e < 0) continue;
+        const int f = e >> 1;
+        val[p] = (T)((e & 1) ? A.up[f] : lw[f]);
+    }
+}
+
+template <typename T>
+void fillDiag(const Layout& L, const HostLdu& A, std::vector<T>& d) {
+    d.resize(L.n);
+    for (int i = 0; i < L.n; ++i) d[i] = (T)A.d[L.perm[i]];
+}
+
+template void fillSellValues<double>(const Layout&, const HostLdu&, bool, std::vector<double>&);
+template void fillSellValues<float>(const Layout&, const HostLdu&, bool, std::vector<float>&);
+template void fillDiag<double>(const Layout&, const HostLdu&, std::vector<double>&);
+template void fillDiag<float>(const Layout&, const HostLdu&, std::vector<float>&);
+
+std::vector<double> dicReciprocalD(const HostLdu& A) {
+    std::vector<double> rD(A.d);
+    const int nf = A.nFaces();
+    for (int f = 0; f < nf; ++f) rD[A.u[f]] -= A.up[f] * A.up[f] / rD[A.l[f]];
+    for (int i = 0; i < A.n; ++i) rD[i] = 1.0 / rD[i];
+    return rD;
+}
+
+std::vector<double> diluReciprocalD(const HostLdu& A) {
+    std::vector<double> rD(A.d);
+    const double* lw = A.lower();
+    const int nf = A.nFaces();
+    for (int f = 0; f < nf; ++f) rD[A.u[f]] -= A.up[f] * lw[f] / rD[A.l[f]];
+    for (int i = 0; i < A.n; ++i) rD[i] = 1.0 / rD[i];
+    return rD;
+}
+
+namespace {
+
+// Pairwise greedy clustering of one level (lduAgglomerate.cpp:94-271, cells
+// visited in ascending order).  Each cell scans its faces as the neighbour
+// first, then as the owner, each in ascending face id.
+std::vector<int> pairCells(const HostLdu& A, const std::vector<double>& w, int& nCoarse) {
+    const int n = A.n, nf = A.nFaces();
+    std::vector<int> off(n + 1, 0);
+    for (int f = 0; f < nf; ++f) {
+        off[A.u[f] + 1]++;
+        off[A.l[f] + 1]++;
+    }
+    for (int c = 0; c < n; ++c) off[c + 1] += off[c];
+    std::vector<int> faces(2 * (size_t)nf), at(off.begin(), off.end() - 1);
+    for (int f = 0; f < nf; ++f) faces[at[A.u[f]]++] = f;
+    for (int f = 0; f < nf; ++f) faces[at[A.l[f]]++] = f;
+
+    std::vector<int> map(n, -1);
+    nCoarse = 0;
+    for (int c = 0; c < n; ++c) {
+        if (map[c] >= 0) continue;
+        int best = -1;
+        double bestW = kSmall;
+        for (int k = off[c]; k < off[c + 1]; ++k) {
+            const int f = faces[k];
+            if (map[A.u[f]] < 0 && map[A.l[f]] < 0 && w[f] > bestW) {
+                best = f;
+                bestW = w[f];
+            }
+        }
+        if (best >= 0) {
+            map[A.u[best]] = nCoarse;
+            map[A.l[best]] = nCoarse;
+            ++nCoarse;
+            continue;
+        }
+        // no free neighbour: join the cluster across the heaviest face
+        int join = -1;
+        double joinW = kSmall;
+        for (int k = off[c]; k < off[c + 1]; ++k) {
+            const int f = faces[k];
+            if (w[f] > joinW) {
+                join = f;
+                joinW = w[f];
+            }
+        }
+        if (join >= 0) map[c] = std::max(map[A.u[join]], map[A.l[join]]);
+    }
+    for (int c = 0; c < n; ++c)
+        if (map[c] < 0) map[c] = nCoarse++;
+    return map;
+}
+
+// Coarse faces from fine faces joining different clusters, numbered owner-major
+// with neighbours in first-appearance order (lduAgglomerationAddressing.cpp:57-178).
+void coarseAddressing(const HostLdu& F, CoarseLevel& C) {
+    const int nf = F.nFaces();
+    const int nc = C.nCoarse;
+    std::vector<std::vector<int>> byOwner(nc);  // provisional face ids per coarse owner
+    std::vector<int> provNei;
+    C.faceRestrictMap.assign(nf, 0);
+    for (int f = 0; f < nf; ++f) {
+        const int a = C.restrictMap[F.u[f]], b = C.restrictMap[F.l[f]];
+        if (a == b) {
+            C.faceRestrictMap[f] = -(a + 1);
+            continue;
+        }
+        const int own = std::min(a, b), nei = std::max(a, b);
+        int hit = -1;
+        for (int k : byOwner[own])
+            if (provNei[k] == nei) {
+                hit = k;
+                break;
+            }
+        if (hit < 0) {
+            hit = (int)provNei.size();
+            byOwner[own].push_back(hit);
+            provNei.push_back(nei);
+        }
+        C.faceRestrictMap[f] = hit;
+    }
+    const int ncf = (int)provNei.size();
+    std::vector<int> renum(ncf);
+    C.A.n = nc;
+    C.A.l.resize(ncf);
+    C.A.u.resize(ncf);
+    int k2 = 0;
+    for (int c = 0; c < nc; ++c)
+        for (int k : byOwner[c]) {
+            C.A.l[k2] = c;
+            C.A.u[k2] = provNei[k];
+            renum[k] = k2++;
+        }
+    for (int f = 0; f < nf; ++f)
+        if (C.faceRestrictMap[f] >= 0) C.faceRestrictMap[f] = renum[C.faceRestrictMap[f]];
+}
+
+// Galerkin coarse coefficients (lduAgglomerateMatrix.cpp:3-104).
+void galerkin(const HostLdu& F, CoarseLevel& C) {
+    HostLdu& A = C.A;
+    const int ncf = A.nFaces();
+    A.d.assign(A.n, 0.0);
+    for (int i = 0; i < F.n; ++i) A.d[C.restrictMap[i]] += F.d[i];
+    const std::vector<int>& fr = C.faceRestrictMap;
+    if (F.symm) {
+        A.symm = true;
+        A.lo.clear();
+        A.up.assign(ncf, 0.0);
+        for (int f = 0; f < F.nFaces(); ++f) {
+            const int cf = fr[f];
+            if (cf >= 0)
+                A.up[cf] += F.up[f];
+            else
+                A.d[-1 - cf] += 2 * F.up[f];
+        }
+        return;
+    }
+    A.symm = false;
+    A.up.assign(ncf, 0.0);
+    A.lo.assign(ncf, 0.0);
+    const double* fl = F.lower();
+    for (int f = 0; f < F.nFaces(); ++f) {
+        const int cf = fr[f];
+        if (cf >= 0) {
+            const int cOwnOfFineOwner = C.restrictMap[F.l[f]];
+            if (A.l[cf] == cOwnOfFineOwner) {
+                A.up[cf] += F.up[f];
+                A.lo[cf] += fl[f];
+            } else if (A.u[cf] == cOwnOfFineOwner) {
+                A.up[cf] += fl[f];
+                A.lo[cf] += F.up[f];
+            } else {
+                throw std::runtime_error("agglomeration: inconsistent coarse addressing");
+            }
+        } else {
+            A.d[-1 - cf] += F.up[f] + fl[f];
+        }
+    }
+}
+
+}  // namespace
+
+std::vector<CoarseLevel> agglomerate(const HostLdu& fine, const std::vector<double>& weights,
+                                     int maxLevels, int nCellsInCoarsestLevel) {
+    std::vector<CoarseLevel> levels;
+    std::vector<double> w(weights);
+    while ((int)levels.size() < maxLevels - 1) {
+        const HostLdu& F = levels.empty() ? fine : levels.back().A;
+        CoarseLevel C;
+        C.restrictMap = pairCells(F, w, C.nCoarse);
+        if (!(C.nCoarse >= nCellsInCoarsestLevel)) break;
+        coarseAddressing(F, C);
+        // face weights of the next level: restrictFaceField (unapMatrix.hpp:441-475)
+        std::vector<double> cw(C.A.nFaces(), 0.0);
+        for (int f = 0; f < F.nFaces(); ++f)
+            if (C.faceRestrictMap[f] >= 0) cw[C.faceRestrictMap[f]] += w[f];
+        w.swap(cw);
+        levels.push_back(std::move(C));
+    }
+    for (size_t i = 0; i < levels.size(); ++i) galerkin(i == 0 ? fine : levels[i - 1].A, levels[i]);
+    return levels;
+}
+
+Transfer buildTransfer(const Layout& fineL, const Layout& coarseL, const std::vector<int>& rm) {
+    Transfer T;
+    const int nf = fineL.n, nc = coarseL.n;
+    T.childStart.assign(nc + 1, 0);
+    for (int e = 0; e < nf; ++e) T.childStart[coarseL.iperm[rm[e]] + 1]++;
+    for (int c = 0; c < nc; ++c) T.childStart[c + 1] += T.childStart[c];
+    T.child.assign(nf, 0);
+    std::vector<int> at(T.childStart.begin(), T.childStart.end() - 1);
+    for (int e = 0; e < nf; ++e) T.child[at[coarseL.iperm[rm[e]]]++] = fineL.iperm[e];
+    T.fineToCoarse.assign(nf, 0);
+    for (int i = 0; i < nf; ++i) T.fineToCoarse[i] = coarseL.iperm[rm[fineL.perm[i]]];
+    return T;
+}
+
+}  // namespace unap_hip

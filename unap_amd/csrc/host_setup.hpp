@@ -1,0 +1,87 @@
+// host_setup.hpp -- host-side setup of the solve path: device layout of an LDU
+// matrix (cell ordering, SELL-64 slices, level schedules), pairwise
+// agglomeration, coarse addressing and Galerkin coarse coefficients.
+//
+// Numbering: "external" ids are the caller's (for coarse levels: the coarse ids
+// the reference's agglomeration would assign); "internal" ids are the device
+// storage order chosen here for HBM locality.  Every per-row arithmetic
+// sequence of the reference is kept: a row lists its faces in ascending face
+// id, so for owner-sorted faces the neighbour-side (losort) terms come first,
+// then the owner-side terms (src/matrix/lduMatrix/lduMatrixSpMV.cpp:87-91).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace unap_hip {
+
+// An LDU matrix in external numbering (the lduMatrix data of the reference,
+// src/matrix/lduMatrix/lduMatrix.hpp:21-42, with 32-bit addressing).
+struct HostLdu {
+    int n = 0;
+    std::vector<int> l, u;         // lowerAddr (owner), upperAddr (neighbour)
+    std::vector<double> lo, up, d; // lower (empty when symmetric), upper, diag
+    bool symm = true;
+    int nFaces() const { return (int)u.size(); }
+    const double* lower() const { return symm ? up.data() : lo.data(); }
+    // faces sorted by owner with owner < neighbour: the order the reference's
+    // sequential sweeps (GS, DIC, DILU) are defined on
+    bool ownerSorted() const;
+};
+
+// Device storage plan of one matrix.
+struct Layout {
+    int n = 0, nf = 0, nslices = 0;
+    long long nnzPadded = 0;
+    std::vector<int> perm;        // internal -> external
+    std::vector<int> iperm;       // external -> internal
+    std::vector<long long> sliceStart;  // nslices + 1, in entries
+    std::vector<uint32_t> rowInfo;      // len | nLowerSide << 16, internal order
+    std::vector<int> col;               // padded, internal ids (padding -> own row)
+    std::vector<int> src;               // padded: 2*face + side (0 lower, 1 upper), -1 pad
+    std::vector<int> fwdStart;          // forward levels: internal rows [fwdStart[k], fwdStart[k+1])
+    std::vector<int> bwdRows;           // internal rows sorted by backward level
+    std::vector<int> bwdStart;          // backward levels: bwdRows[bwdStart[k] .. bwdStart[k+1])
+    int maxRowLen = 0;
+    bool sweepable = false;             // ownerSorted(): sweeps reproduce the reference
+};
+
+Layout buildLayout(const HostLdu& A);
+
+// Values in the device layout.  useUpperBothSides: DIC reads upper on both
+// sweep directions (lduDICPrecond.cpp:68-76) even for asymmetric storage.
+template <typename T>
+void fillSellValues(const Layout& L, const HostLdu& A, bool useUpperBothSides, std::vector<T>& val);
+template <typename T>
+void fillDiag(const Layout& L, const HostLdu& A, std::vector<T>& d);
+
+// rD of DIC / DILU in external numbering (lduDICPrecond.cpp:15-41,
+// lduDILUPrecond.cpp:22-41): sequential face loop, then reciprocal.
+std::vector<double> dicReciprocalD(const HostLdu& A);
+std::vector<double> diluReciprocalD(const HostLdu& A);
+
+// One coarse level of the agglomeration hierarchy.
+struct CoarseLevel {
+    int nCoarse = 0;
+    std::vector<int> restrictMap;      // fine external -> coarse external
+    std::vector<int> faceRestrictMap;  // fine face -> coarse face, or -(coarse cell + 1)
+    HostLdu A;                         // coarse matrix (addressing + Galerkin values)
+};
+
+// lduAgglomeration::agglomerate (lduAgglomerate.cpp:4-91) with the pairwise
+// step (:94-271), coarse addressing (lduAgglomerationAddressing.cpp:8-178)
+// and Galerkin coefficients (lduAgglomerateMatrix.cpp:3-104).
+std::vector<CoarseLevel> agglomerate(const HostLdu& fine, const std::vector<double>& weights,
+                                     int maxLevels, int nCellsInCoarsestLevel);
+
+// Restriction in device order: children of each coarse internal cell as fine
+// internal ids, ascending in fine external id (the summation order of
+// unapMatrix.hpp:403-406), and the fine-internal -> coarse-internal map.
+struct Transfer {
+    std::vector<int> childStart, child;  // CSR over coarse internal cells
+    std::vector<int> fineToCoarse;       // fine internal -> coarse internal
+};
+Transfer buildTransfer(const Layout& fineL, const Layout& coarseL, const std::vector<int>& restrictMap);
+
+}  // namespace unap_hip

@@ -1,0 +1,427 @@
+// kernels.hpp -- CDNA4 (gfx950) kernels of the LDU solve path.
+//
+// Storage: every matrix lives in SELL-64 slices in the internal cell order (see
+// host_setup.hpp).  One thread owns one row; a wavefront owns one 64-row slice,
+// so the k-th coefficient/column load of a wavefront is one contiguous 512 B
+// (FP64) or 256 B (FP32) burst.  Rows accumulate strictly in the reference's
+// order (diagonal first, then faces in ascending face id) with separately
+// rounded multiplies and adds (built with -ffp-contract=off), which makes the
+// FP64 Amul, sweeps and transfers bit-identical to the serial x86 loops.
+//
+// Reductions: every thread keeps FP64 partial sums over a fixed row set, the
+// block reduces them with a fixed wave64 shuffle tree, block totals go to a
+// partials array, and a one-block finalize kernel sums them in index order and
+// runs the solver's scalar step.  Results are run-to-run deterministic.
+//
+// Control: scalar state (rho, alpha, convergence flags, ...) stays on the
+// device; every kernel takes `skip` and returns at once when it is set, so the
+// host can enqueue several iterations ahead and only read back the flags.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace unap_hip {
+
+constexpr int kBlock = 256;      // threads per block of the row kernels
+constexpr int kMaxBlocks = 2048; // partial sums per reduction slot
+constexpr int kMaxSums = 3;
+constexpr double kSmallD = 1.0e-37;   // SMALL (src/base/unap.hpp:88)
+constexpr double kVSmallD = 1.0e-128; // VSMALL (src/base/unap.hpp:89)
+
+// ---------------------------------------------------------------- matrix view
+template <typename T>
+struct DMat {
+    int n;
+    const long long* sliceStart;
+    const uint32_t* rowInfo;  // len | nLowerSide << 16
+    const int* col;
+    const T* val;
+    const T* diag;
+};
+
+// Solver scalar state (one per running Krylov / eigen / MG context).
+struct KState {
+    double rho, rhoOld, alpha, beta, omega, rzo;
+    double initRes, finalRes, prevRes;
+    double tol, relTol;
+    long long maxIter, minIter, nIter;
+    int converged, singular, done, stop;
+    long long exitIt;
+    double* hist;
+    long long histCap;
+    double alphas[32], betas[32];
+    double lambda;
+    double sf[64];      // scaling factors per AMG level (+ finest at [63])
+    double cheb[3 * 64];  // Chebyshev {1/theta, dtemp1, dtemp2} per sweep
+};
+
+// Up to two device flags; a kernel is skipped when either is set.
+struct Skip {
+    const int* a;
+    const int* b;
+    __device__ __forceinline__ bool on() const { return (a && *a) || (b && *b); }
+};
+
+// Logical block index: consecutive logical blocks land on the same XCD (the
+// dispatcher deals workgroups to the 8 XCDs round-robin), so each XCD's L2
+// sees one contiguous row range and its halo instead of the whole vector.
+__device__ __forceinline__ int logicalBlock() {
+    const int p = blockIdx.x, G = gridDim.x;
+    if ((G & 7) == 0) return (p & 7) * (G >> 3) + (p >> 3);
+    return p;
+}
+
+// acc = 0 + d*x[r]; acc += v_k * x[c_k] over the row's faces in order.
+template <typename T, typename TA, typename TX>
+__device__ __forceinline__ TA rowAmul(const DMat<T>& A, int r, const TX* __restrict__ x) {
+    const int s = r >> 6;
+    const long long s0 = A.sliceStart[s];
+    const int slen = (int)((A.sliceStart[s + 1] - s0) >> 6);
+    const int len = (int)(A.rowInfo[r] & 0xffffu);
+    const long long base = s0 + (r & 63);
+    TA acc = (TA)0 + (TA)A.diag[r] * (TA)x[r];
+    for (int k = 0; k < slen; ++k) {
+        const long long p = base + ((long long)k << 6);
+        const TA v = (TA)A.val[p];
+        const TA xv = (TA)x[A.col[p]];
+        if (k < len) acc = acc + v * xv;
+    }
+    return acc;
+}
+
+// ------------------------------------------------------------ reductions
+template <int NS>
+__device__ __forceinline__ void blockReduce(double (&v)[NS], double* sm) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double a = v[s];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
+        v[s] = a;
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) sm[s * 16 + wave] = v[s];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double a = 0.0;
+            for (int w = 0; w < nw; ++w) a += sm[s * 16 + w];
+            v[s] = a;
+        }
+    }
+}
+
+// Rows [begin, end) of this block for a row kernel over n rows with `chunk`
+// rows per logical block.
+struct RowSpan {
+    int begin, end;
+};
+__device__ __forceinline__ RowSpan blockRows(int n, int chunk) {
+    const long long b = logicalBlock();
+    long long beg = b * (long long)chunk;
+    long long end = beg + chunk;
+    if (beg > n) beg = n;
+    if (end > n) end = n;
+    return {(int)beg, (int)end};
+}
+
+// Row kernel with NS FP64 sums.  Body: void operator()(int r, double (&acc)[NS]).
+template <class Body, int NS>
+__global__ void __launch_bounds__(kBlock) kRowsReduce(Body body, int n, int chunk,
+                                                      double* __restrict__ partials, Skip skip) {
+    if (skip.on()) return;
+    __shared__ double sm[kMaxSums * 16];
+    double acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+    const RowSpan sp = blockRows(n, chunk);
+    for (int r = sp.begin + threadIdx.x; r < sp.end; r += blockDim.x) body(r, acc);
+    blockReduce<NS>(acc, sm);
+    if (threadIdx.x == 0) {
+        const int b = logicalBlock();
+#pragma unroll
+        for (int s = 0; s < NS; ++s) partials[s * kMaxBlocks + b] = acc[s];
+    }
+}
+
+// Row kernel without sums.
+template <class Body>
+__global__ void __launch_bounds__(kBlock) kRows(Body body, int n, int chunk, Skip skip) {
+    if (skip.on()) return;
+    const RowSpan sp = blockRows(n, chunk);
+    for (int r = sp.begin + threadIdx.x; r < sp.end; r += blockDim.x) body(r);
+}
+
+// Reference-order reduction: one thread sums body.terms(i) over the caller's
+// cell order (internal id iperm[e] for e = 0, 1, ...), exactly the serial
+// left-to-right loops of the reference (unapVectorOperations.cpp:21-30).
+// Used when bit-identical residual histories are wanted; O(n) sequential.
+template <class Body, int NS>
+__global__ void kSerialSum(Body body, const int* __restrict__ iperm, int n, double* __restrict__ partials,
+                           Skip skip) {
+    if (skip.on()) return;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+    for (int e = 0; e < n; ++e) {
+        double t[NS];
+        body.terms(iperm[e], t);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[s] = acc[s] + t[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) partials[s * kMaxBlocks] = acc[s];
+}
+
+// One block: total of each partials slot in index order, then fin(totals).
+template <class Fin, int NS>
+__global__ void __launch_bounds__(kBlock) kFinalize(Fin fin, const double* __restrict__ partials,
+                                                    int nblocks, Skip skip) {
+    if (skip.on()) return;
+    __shared__ double sm[kMaxSums * 16];
+    double acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double a = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) a += partials[s * kMaxBlocks + b];
+        acc[s] = a;
+    }
+    blockReduce<NS>(acc, sm);
+    if (threadIdx.x == 0) fin(acc);
+}
+
+// ------------------------------------------------------ convergence helpers
+// solverPerformance::checkConvergence (src/matrix/unapMatrixTest.cpp:3-31)
+__device__ __forceinline__ bool checkConv(KState* st, long long iter) {
+    const bool c = ((st->finalRes < st->tol ||
+                     (st->relTol > kSmallD && st->finalRes <= st->relTol * st->initRes)) &&
+                    iter >= st->minIter);
+    st->converged = c ? 1 : 0;
+    return c;
+}
+
+// `++nIter < maxIter && !checkConvergence(..., nIter, ...)` of PCG.cpp:167-180
+// (the check is skipped once maxIter is reached, as the && short-circuits).
+__device__ __forceinline__ void endIteration(KState* st) {
+    st->nIter += 1;
+    if (st->hist && st->nIter < st->histCap) st->hist[st->nIter] = st->finalRes;
+    if (!(st->nIter < st->maxIter))
+        st->done = 1;
+    else if (checkConv(st, st->nIter))
+        st->done = 1;
+}
+
+// ------------------------------------------------------------ state kernels
+__global__ void kStateInit(KState* st, double tol, double relTol, long long maxIter,
+                           long long minIter, double* hist, long long histCap) {
+    if (threadIdx.x != 0) return;
+    st->rho = 1.0e37;  // GREAT (PCG.cpp:57)
+    st->rhoOld = 1.0e37;
+    st->alpha = st->beta = st->omega = st->rzo = 0.0;
+    st->initRes = st->finalRes = st->prevRes = 0.0;
+    st->tol = tol;
+    st->relTol = relTol;
+    st->maxIter = maxIter;
+    st->minIter = minIter;
+    st->nIter = 0;
+    st->converged = st->singular = st->done = st->stop = 0;
+    st->exitIt = -1;
+    st->hist = hist;
+    st->histCap = histCap;
+}
+
+// Initial residual norm.  precheck: MGSolve.cpp:87-96 tests convergence before
+// the first cycle; the Krylov solvers do not.
+struct FinInitRes {
+    KState* st;
+    int precheck;
+    __device__ void operator()(double (&t)[1]) const {
+        st->initRes = sqrt(t[0]) + kSmallD;
+        st->finalRes = st->initRes;
+        st->prevRes = st->initRes;
+        if (st->hist && st->histCap > 0) st->hist[0] = st->initRes;
+        if (precheck && checkConv(st, st->nIter)) st->done = 1;
+    }
+};
+
+// ---------------------------------------------------------------- PCG steps
+// PCG.cpp:100-117: wArA = w.r; singularity test; beta for the p update.
+struct FinPcgRho {
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        st->rhoOld = st->rho;
+        st->rho = t[0];
+        st->singular = (fabs(st->rho) > kSmallD) ? 0 : 1;
+        if (st->singular) {
+            st->done = 1;
+            return;
+        }
+        if (st->nIter != 0) st->beta = st->rho / st->rhoOld;
+    }
+};
+struct FinPcgAlpha {  // PCG.cpp:139-142
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const { st->alpha = st->rho / t[0]; }
+};
+struct FinResNorm {  // PCG.cpp:151 + loop test
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        st->finalRes = sqrt(t[0]) + kSmallD;
+        endIteration(st);
+    }
+};
+
+// ----------------------------------------------------------- PBiCGStab steps
+struct FinBicgRho {  // PBiCGStab.cpp:161-226
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        st->rhoOld = st->rho;
+        st->rho = t[0];
+        st->singular = (fabs(st->rho) > kSmallD) ? 0 : 1;
+        if (st->singular) {
+            st->done = 1;
+            return;
+        }
+        if (st->nIter != 0) st->beta = (st->rho / st->rhoOld) * (st->alpha / st->omega);
+    }
+};
+struct FinBicgAlpha {  // PBiCGStab.cpp:254-269
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const { st->alpha = st->rho / t[0]; }
+};
+struct FinBicgS {  // PBiCGStab.cpp:276-348: converged on ||s|| -> early exit
+    KState* st;
+    long long it;
+    __device__ void operator()(double (&t)[1]) const {
+        st->finalRes = sqrt(t[0]) + kSmallD;
+        if (checkConv(st, st->nIter + 1)) {
+            st->exitIt = it;
+            st->nIter += 1;
+            if (st->hist && st->nIter < st->histCap) st->hist[st->nIter] = st->finalRes;
+            st->done = 1;
+        }
+    }
+};
+struct FinBicgOmega {  // PBiCGStab.cpp:357-386: omega = (t.s) / (t.t)
+    KState* st;
+    __device__ void operator()(double (&t)[2]) const { st->omega = t[1] / t[0]; }
+};
+
+// ---------------------------------------------------------------- FCG steps
+struct FinFcgRz {  // rz = r.z, rzo = r.z_old; beta = (rz - rzo) / rz_old
+    KState* st;
+    __device__ void operator()(double (&t)[2]) const {
+        st->rhoOld = st->rho;
+        st->rho = t[0];
+        st->rzo = t[1];
+        st->singular = (fabs(st->rho) > kSmallD) ? 0 : 1;
+        if (st->singular) {
+            st->done = 1;
+            return;
+        }
+        if (st->nIter != 0) st->beta = (st->rho - st->rzo) / st->rhoOld;
+    }
+};
+
+// ------------------------------------------------------------- eigen steps
+// eigenDiagPCG.cpp:87-123: wArA and the VSMALL break
+struct FinEigRho {
+    KState* st;
+    int it;
+    __device__ void operator()(double (&t)[1]) const {
+        st->rhoOld = st->rho;
+        st->rho = t[0];
+        if (fabs(st->rho) < kVSmallD) {
+            st->stop = 1;
+            return;
+        }
+        if (it == 0) {
+            st->betas[0] = 0.0;
+        } else {
+            st->beta = st->rho / st->rhoOld;
+            st->betas[it] = st->beta;
+        }
+    }
+};
+struct FinEigAlpha {  // eigenDiagPCG.cpp:166-189
+    KState* st;
+    int it;
+    __device__ void operator()(double (&t)[1]) const {
+        st->alpha = st->rho / t[0];
+        st->alphas[it] = st->alpha;
+    }
+};
+
+// computeMaxEig (eigenDiagPCG.cpp:213-315): Lanczos tridiagonal from the CG
+// coefficients, Gershgorin range from [0,0], Sturm bisection to 1e-5.
+__device__ __forceinline__ void chebyTable(KState* st, double boost, double ratio);
+
+__global__ void kLanczosMaxEig(KState* st, int m, double boost, double ratio) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double dg[32], of[32];  // T[i][i], T[i][i+1]
+    for (int i = 0; i < m; ++i) {
+        dg[i] = (i == 0) ? 1.0 / st->alphas[i] : 1.0 / st->alphas[i] + st->betas[i] / st->alphas[i - 1];
+        of[i] = (i < m - 1) ? sqrt(st->betas[i + 1]) / st->alphas[i] : 0.0;
+    }
+    double lo = 0.0, hi = 0.0;
+    for (int i = 0; i < m; ++i) {
+        const double tr = (i == m - 1) ? 0 : of[i];
+        const double tl = (i == 0) ? 0 : of[i - 1];
+        const double lmax = dg[i] + fabs(tr) + fabs(tl);
+        const double lmin = dg[i] - fabs(tr) - fabs(tl);
+        lo = (lo >= lmin) ? lmin : lo;
+        hi = (hi <= lmax) ? lmax : hi;
+    }
+    double xn;
+    do {
+        xn = (lo + hi) * 0.5;
+        int s = 0;
+        double p0 = xn - dg[0];
+        if (p0 < 0) s = 1;
+        double p1 = (xn - dg[1]) * p0 - of[0] * of[0];
+        if (p1 * p0 < 0) s++;
+        for (int k = 2; k < m; ++k) {
+            const double p2 = (xn - dg[k]) * p1 - of[k - 1] * of[k - 1] * p0;
+            if (p2 * p1 < 0) s++;
+            p0 = p1;
+            p1 = p2;
+        }
+        if (s >= 1)
+            lo = xn;
+        else
+            hi = xn;
+    } while ((hi - lo) >= 0.00001);
+    st->lambda = xn;
+    chebyTable(st, boost, ratio);
+}
+
+// Chebyshev coefficients of every sweep from the stored estimate
+// (chebySmoother.cpp:61-111), in the reference's recurrence order.
+__device__ __forceinline__ void chebyTable(KState* st, double boost, double ratio) {
+    const double lam = st->lambda;
+    const double mx = lam * boost, mn = lam / ratio;
+    const double delta = 2.0 / (mx - mn);
+    const double theta = 0.5 * (mx + mn);
+    const double s1 = theta * delta;
+    double rhok = 1.0 / s1;
+    st->cheb[0] = 1.0 / theta;
+    st->cheb[1] = 0.0;
+    st->cheb[2] = 0.0;
+    for (int k = 1; k < 64; ++k) {
+        const double rhokp1 = 1.0 / (2.0 * s1 - rhok);
+        st->cheb[3 * k + 0] = 1.0 / theta;
+        st->cheb[3 * k + 1] = rhokp1 * rhok;
+        st->cheb[3 * k + 2] = 2.0 * rhokp1 * delta;
+        rhok = rhokp1;
+    }
+}
+
+}  // namespace unap_hip

@@ -1,0 +1,584 @@
+// ops.hpp -- the fused row bodies of the solve path and the level-scheduled
+// sweep, transfer and coarsest-level kernels.  Every body cites the reference
+// loop whose per-element arithmetic it keeps.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace unap_hip {
+
+// ============================================================ permutations
+// internal <- external (gather) and external <- internal (scatter), with casts
+template <typename TI, typename TO>
+struct BGather {
+    const int* perm;  // internal -> external
+    const TI* in;
+    TO* out;
+    __device__ void operator()(int i) const { out[i] = (TO)in[perm[i]]; }
+};
+template <typename TI, typename TO>
+struct BScatter {
+    const int* perm;
+    const TI* in;
+    TO* out;
+    __device__ void operator()(int i) const { out[perm[i]] = (TO)in[i]; }
+};
+template <typename T>
+struct BFill {
+    T* out;
+    T v;
+    __device__ void operator()(int i) const { out[i] = v; }
+};
+template <typename TI, typename TO>
+struct BCopy {
+    const TI* in;
+    TO* out;
+    __device__ void operator()(int i) const { out[i] = (TO)in[i]; }
+};
+
+// ================================================================== Amul
+// lduMatrixSpMV.cpp:16-98
+template <typename T>
+struct BAmul {
+    DMat<T> A;
+    const T* x;
+    T* y;
+    __device__ void operator()(int r) const { y[r] = rowAmul<T, T>(A, r, x); }
+};
+
+// r = b - A x (PCG.cpp:61-67), optional shadow copy r0 = r (PBiCGStab.cpp:76-79),
+// sum r*r for normFactor (unapMatrixSolver.cpp:12-16).
+template <typename T>
+struct BResidual {
+    DMat<T> A;
+    const T* x;
+    const T* b;
+    T* r;
+    T* r0;
+    __device__ void operator()(int i, double (&a)[1]) const {
+        const T ax = rowAmul<T, T>(A, i, x);
+        const T ri = b[i] - ax;
+        r[i] = ri;
+        if (r0) r0[i] = ri;
+        a[0] += (double)ri * (double)ri;
+    }
+    __device__ void terms(int i, double (&t)[1]) const { t[0] = (double)r[i] * (double)r[i]; }
+};
+
+// ============================================================ PCG bodies
+// w = rD * r and w.r (lduDiagPrecond.cpp:56-59 fused with PCG.cpp:105)
+struct BDiagDot {
+    const double* rD;
+    const double* r;
+    double* w;
+    __device__ void operator()(int i, double (&a)[1]) const {
+        const double wi = rD[i] * r[i];
+        w[i] = wi;
+        a[0] += wi * r[i];
+    }
+    __device__ void terms(int i, double (&t)[1]) const { t[0] = w[i] * r[i]; }
+};
+// w = r (identity preconditioner) and w.r
+struct BCopyDot {
+    const double* r;
+    double* w;
+    __device__ void operator()(int i, double (&a)[1]) const {
+        const double wi = r[i];
+        w[i] = wi;
+        a[0] += wi * r[i];
+    }
+    __device__ void terms(int i, double (&t)[1]) const { t[0] = w[i] * r[i]; }
+};
+struct BDot {
+    const double* a;
+    const double* b;
+    __device__ void operator()(int i, double (&s)[1]) const { s[0] += a[i] * b[i]; }
+    __device__ void terms(int i, double (&t)[1]) const { t[0] = a[i] * b[i]; }
+};
+// PCG.cpp:119-134
+struct BPcgP {
+    const double* w;
+    double* p;
+    const KState* st;
+    __device__ void operator()(int i) const {
+        if (st->nIter == 0)
+            p[i] = w[i];
+        else
+            p[i] = w[i] + st->beta * p[i];
+    }
+};
+// w = A p and w.p (PCG.cpp:137-139)
+struct BAmulDot {
+    DMat<double> A;
+    const double* p;
+    double* w;
+    __device__ void operator()(int i, double (&a)[1]) const {
+        const double wi = rowAmul<double, double>(A, i, p);
+        w[i] = wi;
+        a[0] += wi * p[i];
+    }
+    __device__ void terms(int i, double (&t)[1]) const { t[0] = w[i] * p[i]; }
+};
+// x += alpha p; r -= alpha w; sum r*r (PCG.cpp:144-151)
+struct BPcgXR {
+    double* x;
+    double* r;
+    const double* p;
+    const double* w;
+    const KState* st;
+    __device__ void operator()(int i, double (&a)[1]) const {
+        const double al = st->alpha;
+        x[i] += al * p[i];
+        const double ri = r[i] - al * w[i];
+        r[i] = ri;
+        a[0] += ri * ri;
+    }
+    __device__ void terms(int i, double (&t)[1]) const { t[0] = r[i] * r[i]; }
+};
+
+// ====================================================== PBiCGStab bodies
+struct BBicgP {  // PBiCGStab.cpp:131-175
+    const double* r;
+    const double* Ay;
+    double* p;
+    const KState* st;
+    __device__ void operator()(int i) const {
+        if (st->nIter == 0)
+            p[i] = r[i];
+        else
+            p[i] = r[i] + st->beta * (p[i] - st->omega * Ay[i]);
+    }
+};
+struct BAmulDotR0 {  // AyA = A yA; rA0.AyA (PBiCGStab.cpp:248-254)
+    DMat<double> A;
+    const double* y;
+    const double* r0;
+    double* Ay;
+    __device__ void operator()(int i, double (&a)[1]) const {
+        const double v = rowAmul<double, double>(A, i, y);
+        Ay[i] = v;
+        a[0] += r0[i] * v;
+    }
+    __device__ void terms(int i, double (&t)[1]) const { t[0] = r0[i] * Ay[i]; }
+};
+struct BBicgS {  // sA = rA - alpha AyA; ||sA|| (PBiCGStab.cpp:276-281)
+    const double* r;
+    const double* Ay;
+    double* s;
+    const KState* st;
+    __device__ void operator()(int i, double (&a)[1]) const {
+        const double si = r[i] - st->alpha * Ay[i];
+        s[i] = si;
+        a[0] += si * si;
+    }
+    __device__ void terms(int i, double (&q)[1]) const { q[0] = s[i] * s[i]; }
+};
+struct BAmulTT {  // tA = A zA; tA.tA and tA.sA (PBiCGStab.cpp:355-364)
+    DMat<double> A;
+    const double* z;
+    const double* s;
+    double* t;
+    __device__ void operator()(int i, double (&a)[2]) const {
+        const double ti = rowAmul<double, double>(A, i, z);
+        t[i] = ti;
+        a[0] += ti * ti;
+        a[1] += ti * s[i];
+    }
+    __device__ void terms(int i, double (&q)[2]) const {
+        q[0] = t[i] * t[i];
+        q[1] = t[i] * s[i];
+    }
+};
+struct BBicgXR {  // PBiCGStab.cpp:388-396
+    double* x;
+    double* r;
+    const double* y;
+    const double* z;
+    const double* s;
+    const double* t;
+    const KState* st;
+    __device__ void operator()(int i, double (&a)[1]) const {
+        const double al = st->alpha, om = st->omega;
+        x[i] += al * y[i] + om * z[i];
+        const double ri = s[i] - om * t[i];
+        r[i] = ri;
+        a[0] += ri * ri;
+    }
+    __device__ void terms(int i, double (&q)[1]) const { q[0] = r[i] * r[i]; }
+};
+// early exit x += alpha yA (PBiCGStab.cpp:298-312), only in the iteration that
+// took it
+__global__ void kBicgEarlyX(double* x, const double* y, const KState* st, long long it, int n) {
+    if (st->exitIt != it) return;
+    const double al = st->alpha;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        x[i] += al * y[i];
+}
+
+// ================================================================ FCG bodies
+// rz = r.z and rzo = r.z_old with z in the preconditioner's precision
+template <typename TZ>
+struct BFcgRz {
+    const double* r;
+    const TZ* z;
+    const TZ* zOld;
+    int first;
+    __device__ void operator()(int i, double (&a)[2]) const {
+        const double ri = r[i];
+        a[0] += ri * (double)z[i];
+        if (!first) a[1] += ri * (double)zOld[i];
+    }
+    __device__ void terms(int i, double (&t)[2]) const {
+        t[0] = r[i] * (double)z[i];
+        t[1] = first ? 0.0 : r[i] * (double)zOld[i];
+    }
+};
+template <typename TZ>
+struct BFcgP {
+    const TZ* z;
+    double* p;
+    const KState* st;
+    __device__ void operator()(int i) const {
+        if (st->nIter == 0)
+            p[i] = (double)z[i];
+        else
+            p[i] = (double)z[i] + st->beta * p[i];
+    }
+};
+
+// ============================================================== MG bodies
+// Acf = A f; sums source.f and Acf.f (scalingFactor.cpp:19-43, first half)
+template <typename T>
+struct BSfAmul {
+    DMat<T> A;
+    const T* f;
+    const T* src;
+    T* Acf;
+    __device__ void operator()(int i, double (&a)[2]) const {
+        const T v = rowAmul<T, T>(A, i, f);
+        Acf[i] = v;
+        const T fi = f[i];
+        a[0] += (double)(src[i] * fi);
+        a[1] += (double)(v * fi);
+    }
+    __device__ void terms(int i, double (&t)[2]) const {
+        t[0] = (double)(src[i] * f[i]);
+        t[1] = (double)(Acf[i] * f[i]);
+    }
+};
+struct FinSF {  // scalingFactor.cpp:63-64
+    KState* st;
+    int slot;
+    __device__ void operator()(double (&t)[2]) const { st->sf[slot] = t[0] / (t[1] + kSmallD); }
+};
+// field += (source - Acf)/D (scalingFactor.cpp:42), then field *= sf (Vcycle.cpp:82-88,206-212)
+template <typename T>
+struct BJacobiScale {
+    T* f;
+    const T* src;
+    const T* Acf;
+    const T* d;
+    const KState* st;
+    int slot;
+    const T* pre;  // pre-smoothed correction to add back (Vcycle.cpp:224-243), or null
+    __device__ void operator()(int i) const {
+        const T sf = (T)st->sf[slot];
+        T fi = f[i] + (src[i] - Acf[i]) / d[i];
+        fi = fi * sf;
+        if (pre) fi = fi + pre[i];
+        f[i] = fi;
+    }
+};
+// psi += fsf * (finestCorrection + (finestResidual - Apsi)/D) (Vcycle.cpp:266-286
+// with the Jacobi update of scalingFactor.cpp:42)
+template <typename T>
+struct BPsiScaled {
+    T* psi;
+    T* fc;
+    const T* fres;
+    const T* Apsi;
+    const T* d;
+    const KState* st;
+    int slot;
+    __device__ void operator()(int i) const {
+        const T fsf = (T)st->sf[slot];
+        const T c = fc[i] + (fres[i] - Apsi[i]) / d[i];
+        fc[i] = c;
+        psi[i] = psi[i] + fsf * c;
+    }
+};
+template <typename T>
+struct BAddTo {  // a += b (Vcycle.cpp:300-311)
+    T* a;
+    const T* b;
+    __device__ void operator()(int i) const { a[i] = a[i] + b[i]; }
+};
+template <typename T>
+struct BSubAmul {  // src -= A corr (Vcycle.cpp:100-124)
+    DMat<T> A;
+    const T* x;
+    T* src;
+    __device__ void operator()(int i) const {
+        const T v = rowAmul<T, T>(A, i, x);
+        src[i] = src[i] - v;
+    }
+};
+template <typename T>
+struct BResidNoNorm {  // finestResidual = b - Apsi after Apsi = A psi (MGSolve.cpp:127-132)
+    DMat<T> A;
+    const T* x;
+    const T* b;
+    T* Apsi;
+    T* r;
+    __device__ void operator()(int i) const {
+        const T v = rowAmul<T, T>(A, i, x);
+        Apsi[i] = v;
+        r[i] = b[i] - v;
+    }
+};
+
+// restrictField (unapMatrix.hpp:378-407): children summed in ascending fine
+// external id; optionally also writes the fine field cast to TC.
+template <typename TF, typename TC>
+struct BRestrict {
+    const int* cs;
+    const int* ch;
+    const TF* ff;
+    TC* cf;
+    TC* ffCast;
+    __device__ void operator()(int c) const {
+        TC acc = (TC)0;
+        for (int k = cs[c]; k < cs[c + 1]; ++k) {
+            const int i = ch[k];
+            const TF v = ff[i];
+            acc = acc + (TC)v;
+            if (ffCast) ffCast[i] = (TC)v;
+        }
+        cf[c] = acc;
+    }
+};
+// prolongField (unapMatrix.hpp:410-438)
+template <typename T>
+struct BProlong {
+    const int* map;
+    const T* cf;
+    T* ff;
+    __device__ void operator()(int i) const { ff[i] = cf[map[i]]; }
+};
+
+// ============================================================ Chebyshev
+// chebySmoother.cpp:73-88 (sweep 0) and :105-144 (later sweeps), one sweep per
+// kernel, Amul fused; x is ping-ponged (x_in -> x_out) because neighbours
+// still need the old values.  rD = 1/D is recomputed (the smoother builds a
+// lduDiagPrecond on every call, chebySmoother.cpp:21).
+template <typename T>
+struct BCheby {
+    DMat<T> A;
+    const T* xin;
+    T* xout;
+    const T* b;
+    T* w;
+    const KState* est;  // coefficient table
+    int sweep;
+    __device__ void operator()(int i) const {
+        const T ax = rowAmul<T, T>(A, i, xin);
+        const T rd = (T)1 / A.diag[i];
+        T wi;
+        if (sweep == 0) {
+            wi = rd * (b[i] - ax) * (T)est->cheb[0];
+        } else {
+            wi = (T)est->cheb[3 * sweep + 1] * w[i] + (T)est->cheb[3 * sweep + 2] * rd * (b[i] - ax);
+        }
+        w[i] = wi;
+        xout[i] = xin[i] + wi;
+    }
+};
+
+// ============================================================ eigenDiagPCG
+// eigenDiagPCG.cpp:87-111 w = rD r, wArA; :124-165 p update; :166-186 w = A p,
+// wApA; :190-197 x, r update.  rD = 1/D.
+template <typename T>
+struct BEigW {
+    const T* d;
+    const T* r;
+    T* w;
+    __device__ void operator()(int i, double (&a)[1]) const {
+        const T wi = ((T)1 / d[i]) * r[i];
+        w[i] = wi;
+        a[0] += (double)(wi * r[i]);
+    }
+    __device__ void terms(int i, double (&t)[1]) const { t[0] = (double)(w[i] * r[i]); }
+};
+template <typename T>
+struct BEigP {
+    const T* w;
+    T* p;
+    const KState* st;
+    int it;
+    __device__ void operator()(int i) const {
+        if (it == 0)
+            p[i] = w[i];
+        else
+            p[i] = w[i] + (T)st->beta * p[i];
+    }
+};
+template <typename T>
+struct BEigAp {
+    DMat<T> A;
+    const T* p;
+    T* w;
+    __device__ void operator()(int i, double (&a)[1]) const {
+        const T v = rowAmul<T, T>(A, i, p);
+        w[i] = v;
+        a[0] += (double)(v * p[i]);
+    }
+    __device__ void terms(int i, double (&t)[1]) const { t[0] = (double)(w[i] * p[i]); }
+};
+template <typename T>
+struct BEigXR {
+    T* x;
+    T* r;
+    const T* p;
+    const T* w;
+    const KState* st;
+    __device__ void operator()(int i) const {
+        const T al = (T)st->alpha;
+        x[i] += al * p[i];
+        r[i] -= al * w[i];
+    }
+};
+template <typename T>
+struct BResidPlain {  // r = b - A x (eigenDiagPCG.cpp:64-76)
+    DMat<T> A;
+    const T* x;
+    const T* b;
+    T* r;
+    __device__ void operator()(int i) const { r[i] = b[i] - rowAmul<T, T>(A, i, x); }
+};
+
+// ================================================================ sweeps
+// Row helpers.  e(k) = position of entry k of row r.
+template <typename T>
+struct RowEnt {
+    long long base;
+    uint32_t info;
+    __device__ RowEnt(const DMat<T>& A, int r) {
+        base = A.sliceStart[r >> 6] + (r & 63);
+        info = A.rowInfo[r];
+    }
+    __device__ int len() const { return (int)(info & 0xffffu); }
+    __device__ int mid() const { return (int)(info >> 16); }
+    __device__ long long at(int k) const { return base + ((long long)k << 6); }
+};
+
+// Gauss-Seidel row (lduGaussSeidelSmoother.cpp:53-78): b minus the lower-side
+// terms (new x) then the upper-side terms (old x), then / diag.
+template <typename T>
+__device__ __forceinline__ void gsRow(const DMat<T>& A, int r, T* x, const T* b) {
+    RowEnt<T> e(A, r);
+    T acc = b[r];
+    const int len = e.len();
+    for (int k = 0; k < len; ++k) {
+        const long long p = e.at(k);
+        acc = acc - A.val[p] * x[A.col[p]];
+    }
+    acc = acc / A.diag[r];
+    x[r] = acc;
+}
+// DIC/DILU forward row (lduDICPrecond.cpp:141-149, lduDILUPrecond.cpp:223-235):
+// w = rD r, then w -= (rD * coef) * w[lower neighbour] in losort order.
+template <typename T, typename TV>
+__device__ __forceinline__ void fwdRow(const DMat<T>& A, int r, TV* w, const TV* rr, const TV* rD) {
+    RowEnt<T> e(A, r);
+    const TV rd = rD[r];
+    TV acc = rd * rr[r];
+    const int mid = e.mid();
+    for (int k = 0; k < mid; ++k) {
+        const long long p = e.at(k);
+        acc = acc - (rd * (TV)A.val[p]) * w[A.col[p]];
+    }
+    w[r] = acc;
+}
+// backward row (lduDICPrecond.cpp:151-154): owner faces in descending order
+template <typename T, typename TV>
+__device__ __forceinline__ void bwdRow(const DMat<T>& A, int r, TV* w, const TV* rD) {
+    RowEnt<T> e(A, r);
+    const TV rd = rD[r];
+    TV acc = w[r];
+    const int mid = e.mid();
+    for (int k = e.len() - 1; k >= mid; --k) {
+        const long long p = e.at(k);
+        acc = acc - (rd * (TV)A.val[p]) * w[A.col[p]];
+    }
+    w[r] = acc;
+}
+
+enum SweepMode { SWEEP_GS = 0, SWEEP_FWD = 1, SWEEP_BWD = 2 };
+
+template <typename T, typename TV>
+struct SweepArgs {
+    DMat<T> A;  // val: lower-side coefficients for GS / DILU-forward, upper for DIC / backward
+    TV* x;      // GS: x;  FWD/BWD: w
+    const TV* b;   // GS: b;  FWD: r
+    const TV* rD;  // FWD/BWD
+    const int* rows;      // backward: row list, null = contiguous
+    const int* levStart;  // level boundaries (device)
+};
+
+template <typename T, typename TV, int MODE>
+__device__ __forceinline__ void sweepRow(const SweepArgs<T, TV>& a, int r) {
+    if (MODE == SWEEP_GS)
+        gsRow<T>(a.A, r, a.x, a.b);
+    else if (MODE == SWEEP_FWD)
+        fwdRow<T, TV>(a.A, r, a.x, a.b, a.rD);
+    else
+        bwdRow<T, TV>(a.A, r, a.x, a.rD);
+}
+
+// One level: positions [p0, p1) of the schedule.
+template <typename T, typename TV, int MODE>
+__global__ void __launch_bounds__(kBlock) kSweepLevel(SweepArgs<T, TV> a, int p0, int p1, Skip sk) {
+    if (sk.on()) return;
+    const int q = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= p1) return;
+    const int r = a.rows ? a.rows[q] : q;
+    sweepRow<T, TV, MODE>(a, r);
+}
+
+// Levels [l0, l1) in one workgroup (the narrow tail of a schedule, or a whole
+// small matrix), nSweeps times (GS only; sweeps re-read b every time).
+template <typename T, typename TV, int MODE>
+__global__ void __launch_bounds__(1024) kSweepTail(SweepArgs<T, TV> a, int l0, int l1, int nSweeps,
+                                                   Skip sk) {
+    if (sk.on()) return;
+    for (int s = 0; s < nSweeps; ++s) {
+        for (int k = l0; k < l1; ++k) {
+            const int p0 = a.levStart[k], p1 = a.levStart[k + 1];
+            for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
+                const int r = a.rows ? a.rows[q] : q;
+                sweepRow<T, TV, MODE>(a, r);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ================================================== dense coarsest (FP64)
+// y = Ainv b for the coarsest level of the mixed-precision hierarchy: one
+// wavefront per row, coalesced row reads, FP64 accumulation.
+template <typename TB>
+__global__ void __launch_bounds__(256) kDenseSolve(const double* __restrict__ Ainv, const TB* b, TB* y,
+                                                   int n, Skip sk) {
+    if (sk.on()) return;
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const double* a = Ainv + (size_t)row * n;
+    double acc = 0.0;
+    for (int j = lane; j < n; j += 64) acc += a[j] * (double)b[j];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) y[row] = (TB)acc;
+}
+
+}  // namespace unap_hip

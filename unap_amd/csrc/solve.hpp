@@ -1,0 +1,247 @@
+// solve.hpp -- host orchestration of the solve path: matrices, preconditioners,
+// smoothers, the agglomeration hierarchy with its V-cycle, and the Krylov/MG
+// drivers.  All vectors handled here are device vectors in internal order.
+#pragma once
+
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "coarse.hpp"
+#include "device.hpp"
+
+namespace unap_hip {
+
+// ------------------------------------------------------------------ matrix
+struct LduObj {
+    HostLdu host;
+    Layout layout;
+    std::unique_ptr<DevMatrix<double>> d64;
+    std::unique_ptr<DevMatrix<float>> d32;
+    DevMatrix<double>& dev64() {
+        if (!d64) {
+            d64.reset(new DevMatrix<double>);
+            d64->build(host, layout, Ctx::get().stream);
+        }
+        return *d64;
+    }
+    DevMatrix<float>& dev32() {
+        if (!d32) {
+            d32.reset(new DevMatrix<float>);
+            d32->build(host, layout, Ctx::get().stream);
+        }
+        return *d32;
+    }
+    int n() const { return host.n; }
+};
+
+// Levels of a schedule whose rows are few enough to run in one workgroup:
+// the first level index from which every level holds at most kTailRows rows.
+constexpr int kTailRows = 2048;
+inline int tailStart(const std::vector<int>& start) {
+    const int nl = (int)start.size() - 1;
+    int k = nl;
+    while (k > 0 && start[k] - start[k - 1] <= kTailRows) --k;
+    return k;
+}
+
+// One pass over a level schedule: wide levels one kernel each, the narrow
+// tail in one workgroup.
+template <typename T, typename TV, int MODE>
+void runSchedule(SweepArgs<T, TV> a, const std::vector<int>& levStartH, Skip sk) {
+    Ctx& c = Ctx::get();
+    const int nl = (int)levStartH.size() - 1;
+    const int tail = tailStart(levStartH);
+    for (int k = 0; k < tail; ++k) {
+        const int p0 = levStartH[k], p1 = levStartH[k + 1];
+        if (p1 <= p0) continue;
+        const int g = (p1 - p0 + kBlock - 1) / kBlock;
+        kSweepLevel<T, TV, MODE><<<g, kBlock, 0, c.stream>>>(a, p0, p1, sk);
+        UH_LAUNCHED();
+    }
+    if (tail < nl) {
+        kSweepTail<T, TV, MODE><<<1, 1024, 0, c.stream>>>(a, tail, nl, 1, sk);
+        UH_LAUNCHED();
+    }
+}
+
+// Gauss-Seidel smoothing, nSweeps forward sweeps (lduGaussSeidelSmoother.cpp:40-79).
+template <typename T>
+void gsSmooth(DevMatrix<T>& M, T* x, const T* b, int nSweeps, Skip sk) {
+    if (!M.sweepable) throw Error(-3, "Gauss-Seidel needs faces sorted by owner with owner < neighbour");
+    SweepArgs<T, T> a{M.view(), x, b, nullptr, nullptr, M.fwdStart.get()};
+    const int nl = (int)M.fwdStartH.size() - 1;
+    if (tailStart(M.fwdStartH) == 0) {
+        kSweepTail<T, T, SWEEP_GS><<<1, 1024, 0, Ctx::get().stream>>>(a, 0, nl, nSweeps, sk);
+        UH_LAUNCHED();
+        return;
+    }
+    for (int s = 0; s < nSweeps; ++s) runSchedule<T, T, SWEEP_GS>(a, M.fwdStartH, sk);
+}
+
+// DIC / DILU application w = M^-1 r: forward then backward schedule.
+template <typename T, typename TV>
+void luSweeps(DevMatrix<T>& M, bool dic, TV* w, const TV* r, const TV* rD, Skip sk) {
+    if (!M.sweepable) throw Error(-3, "DIC/DILU needs faces sorted by owner with owner < neighbour");
+    const DMat<T> Av = dic ? M.viewU() : M.view();
+    SweepArgs<T, TV> f{Av, w, r, rD, nullptr, M.fwdStart.get()};
+    runSchedule<T, TV, SWEEP_FWD>(f, M.fwdStartH, sk);
+    SweepArgs<T, TV> bk{Av, w, nullptr, rD, M.bwdRows.get(), M.bwdStart.get()};
+    runSchedule<T, TV, SWEEP_BWD>(bk, M.bwdStartH, sk);
+}
+
+// Work vectors of one level.
+template <typename T>
+struct LevelWork {
+    DBuf<T> a, b, c, d, e;
+    void ensure(int n) {
+        if ((int)a.n != n) {
+            a.alloc(n);
+            b.alloc(n);
+            c.alloc(n);
+            d.alloc(n);
+            e.alloc(n);
+        }
+    }
+};
+
+// Chebyshev smoothing (chebySmoother.cpp:13-149).  est holds the coefficient
+// table.  x is ping-ponged through tmp; the result ends in x.
+template <typename T>
+void chebySweeps(DevMatrix<T>& M, T* x, const T* b, int nSweeps, const KState* est, T* w, T* tmp,
+                 Skip sk) {
+    if (nSweeps > 64) throw Error(-1, "Chebyshev: at most 64 sweeps per call");
+    T* cur = x;
+    T* nxt = tmp;
+    for (int s = 0; s < nSweeps; ++s) {
+        launchRows(BCheby<T>{M.view(), cur, nxt, b, w, est, s}, M.n, sk);
+        std::swap(cur, nxt);
+    }
+    if (cur != x)
+        UH_CHECK(hipMemcpyAsync(x, cur, sizeof(T) * M.n, hipMemcpyDeviceToDevice, Ctx::get().stream));
+}
+
+// eigenDiagPCG (eigenDiagPCG.cpp:41-212): nPCG Jacobi-PCG steps updating x,
+// then the Lanczos estimate and the Chebyshev coefficient table.
+template <typename T>
+void eigenEstimate(DevMatrix<T>& M, T* x, const T* b, int nPCG, KState* est, T* p, T* w, T* r,
+                   Skip sk, double boost = 1.1, double ratio = 1.5) {
+    if (nPCG < 2 || nPCG > 32) throw Error(-1, "eigen estimate: 2..32 PCG steps");
+    Ctx& c = Ctx::get();
+    kStateInit<<<1, 1, 0, c.stream>>>(est, 0, 0, 0, 0, nullptr, 0);
+    UH_LAUNCHED();
+    launchRows(BResidPlain<T>{M.view(), x, b, r}, M.n, sk);
+    const Skip sk2{sk.a, &est->stop};
+    for (int it = 0; it < nPCG; ++it) {
+        launchReduce<1>(BEigW<T>{M.diag.get(), r, w}, M.n, M.iperm.get(), FinEigRho{est, it}, sk2);
+        launchRows(BEigP<T>{w, p, est, it}, M.n, sk2);
+        launchReduce<1>(BEigAp<T>{M.view(), p, w}, M.n, M.iperm.get(), FinEigAlpha{est, it}, sk2);
+        launchRows(BEigXR<T>{x, r, p, w, est}, M.n, sk2);
+    }
+    kLanczosMaxEig<<<1, 1, 0, c.stream>>>(est, nPCG, boost, ratio);
+    UH_LAUNCHED();
+}
+
+// ----------------------------------------------------------------- smoother
+struct SmootherObj {
+    int type;                // 1 GS, 2 Chebyshev
+    bool firstCall = true;   // Chebyshev: next call only estimates (chebySmoother.cpp:22-42)
+    DBuf<KState> est;
+    SmootherObj(int t) : type(t), est(1) {}
+};
+
+// ----------------------------------------------------- agglomeration levels
+struct AgglHost {
+    std::vector<CoarseLevel> levels;
+    std::vector<Layout> layouts;
+};
+inline void buildAggl(const HostLdu& fine, const std::vector<double>& w, int maxLevels, int nCoarsest,
+                      AgglHost& out) {
+    out.levels = agglomerate(fine, w, maxLevels, nCoarsest);
+    out.layouts.clear();
+    for (auto& L : out.levels) out.layouts.push_back(buildLayout(L.A));
+}
+
+// -------------------------------------------------------------- hierarchy
+struct AmgParams {
+    int smoother = 1, nPre = 0, nPost = 2, nFinest = 2, nVcycles = 2, precision = 64;
+    int maxLevels = 50, nCoarsest = 50;
+};
+
+// The V-cycle of MGSolver (Vcycle.cpp:27-332) in precision T on a resident
+// hierarchy.  For T = float the coarsest level is solved directly in FP64.
+template <typename T>
+struct Hierarchy {
+    LduObj* fineObj = nullptr;
+    DevMatrix<T>* fine = nullptr;
+    std::shared_ptr<AgglHost> ag;
+    std::vector<std::unique_ptr<DevMatrix<T>>> mats;  // coarse levels (not the coarsest for FP32)
+    std::vector<DBuf<int>> cs, ch, f2c;                // transfer into level l from its fine level
+    std::vector<LevelWork<T>> work;                   // corr(a) src(b) t1(c) t2(d) per coarse level
+    LevelWork<T> fw;                                  // finest: fCorr(a) Apsi(b) w(c) tmp(d)
+    DBuf<T> fres, pre0, src32;                        // finest residual copy, spare, source cast
+    DBuf<double> coarseSrc64, coarseCorr64;           // FP32 path: coarsest in FP64
+    std::vector<DBuf<T>> pre;                         // pre-smoothed corrections (nPre > 0)
+    AmgParams prm;
+    bool scale = true;
+    int L = 0;
+    // smoothers: index 0 finest, l+1 coarse level l
+    std::vector<char> firstCall;
+    DBuf<KState> est;    // one per smoother
+    DBuf<KState> mgst;   // scaling factors
+    // coarsest (FP64 reference path)
+    std::unique_ptr<DevMatrix<double>> coarsest64;
+    DBuf<double> rDc;
+    size_t coarseLds = 0;
+    // coarsest (FP32 path): dense inverse in internal order
+    DBuf<double> Ainv;
+
+    int nl(int l) const { return l < 0 ? fine->n : ag->levels[l].nCoarse; }
+    DevMatrix<T>& mat(int l) { return l < 0 ? *fine : *mats[l]; }
+
+    void build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgParams& p);
+    void initSmoothers() {
+        for (auto& f : firstCall) f = 1;
+    }
+    void smooth(int si, T* x, const T* b, int nSweeps, Skip sk);
+    void coarsestSolve(Skip sk);
+    void vcycle(T* psi, const T* source, T* fRes, const double* fRes64, Skip sk);
+    void precondition(const double* r, T* w, Skip sk);
+};
+
+// ------------------------------------------------------------ preconditioner
+struct PrecondObj {
+    int type = 0;
+    LduObj* A = nullptr;
+    DBuf<double> rD;                 // internal order
+    std::vector<double> rDext;       // external order (rD())
+    std::unique_ptr<Hierarchy<double>> amg64;
+    std::unique_ptr<Hierarchy<float>> amg32;
+    int precision() const { return amg32 ? 32 : 64; }
+};
+
+// precondition(w, r) in FP64 (types 0-3 and the FP64 hierarchy).
+void applyPrecond64(PrecondObj& P, const double* r, double* w, Skip sk);
+
+// ----------------------------------------------------------------- solvers
+struct SolverObj {
+    int type = 1;
+    PrecondObj* P = nullptr;
+    long long maxIter = 1000, minIter = 0;
+    double relTol = 1e-6, tol = 0.0;
+    // standalone MG
+    AmgParams mg;
+    bool mgSet = false;
+    std::vector<double> mgWeights;
+    bool mgHasWeights = false;
+    const double* mgWeightsPtr = nullptr;  // caller-owned, read at the next solve
+    std::unique_ptr<Hierarchy<double>> mgH;
+    LduObj* mgFor = nullptr;
+    // results
+    std::vector<double> history;
+    double lastMs = 0;
+};
+
+}  // namespace unap_hip

@@ -1,0 +1,1165 @@
+// unap_hip.hip -- the solve path on the GPU and its C-ABI (include/unap_hip.h).
+#include "unap_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+
+#include "solve.hpp"
+
+namespace unap_hip {
+
+static thread_local std::string g_lastError;
+
+Ctx& Ctx::get() {
+    static Ctx* c = nullptr;
+    if (!c) {
+        Ctx* n = new Ctx;
+        UH_CHECK(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
+        n->partials.alloc((size_t)kMaxSums * kMaxBlocks);
+        n->scratchState.alloc(4);
+        c = n;
+    }
+    return *c;
+}
+
+// Pinned host mirror of one KState for flag read-back.
+static KState* hostState() {
+    static KState* h = nullptr;
+    if (!h) UH_CHECK(hipHostMalloc((void**)&h, sizeof(KState), hipHostMallocDefault));
+    return h;
+}
+static const KState& readState(const KState* d) {
+    Ctx& c = Ctx::get();
+    KState* h = hostState();
+    UH_CHECK(hipMemcpyAsync(h, d, sizeof(KState), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    return *h;
+}
+
+// ---------------------------------------------------------------- hierarchy
+template <typename T>
+struct Fine;
+template <>
+struct Fine<double> {
+    static DevMatrix<double>& get(LduObj& A) { return A.dev64(); }
+};
+template <>
+struct Fine<float> {
+    static DevMatrix<float>& get(LduObj& A) { return A.dev32(); }
+};
+
+// Dense inverse by Gauss-Jordan elimination with partial pivoting (host, FP64).
+static std::vector<double> denseInverse(std::vector<double> a, int n) {
+    std::vector<double> inv((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        double best = std::fabs(a[(size_t)c * n + c]);
+        for (int r = c + 1; r < n; ++r) {
+            const double v = std::fabs(a[(size_t)r * n + c]);
+            if (v > best) {
+                best = v;
+                piv = r;
+            }
+        }
+        if (best == 0.0) throw Error(-4, "coarsest level matrix is singular");
+        if (piv != c) {
+            for (int j = 0; j < n; ++j) {
+                std::swap(a[(size_t)c * n + j], a[(size_t)piv * n + j]);
+                std::swap(inv[(size_t)c * n + j], inv[(size_t)piv * n + j]);
+            }
+        }
+        const double d = 1.0 / a[(size_t)c * n + c];
+        for (int j = 0; j < n; ++j) {
+            a[(size_t)c * n + j] *= d;
+            inv[(size_t)c * n + j] *= d;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = a[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < n; ++j) {
+                a[(size_t)r * n + j] -= f * a[(size_t)c * n + j];
+                inv[(size_t)r * n + j] -= f * inv[(size_t)c * n + j];
+            }
+        }
+    }
+    return inv;
+}
+
+template <typename T>
+void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgParams& p) {
+    Ctx& c = Ctx::get();
+    fineObj = &A;
+    fine = &Fine<T>::get(A);
+    ag = agg;
+    prm = p;
+    L = (int)ag->levels.size();
+    if (L == 0)
+        throw Error(-4, "agglomeration produced no coarse level (matrix smaller than the coarsest-level size)");
+    scale = A.host.symm;  // scaleCorrection_(A.symm()), MG.cpp:14
+    const bool fp32 = sizeof(T) == 4;
+    mats.clear();
+    cs.clear();
+    ch.clear();
+    f2c.clear();
+    work.clear();
+    mats.resize(L);
+    work.resize(L);
+    for (int l = 0; l < L; ++l) {
+        const Layout& fl = (l == 0) ? A.layout : ag->layouts[l - 1];
+        const Layout& cl = ag->layouts[l];
+        Transfer tr = buildTransfer(fl, cl, ag->levels[l].restrictMap);
+        cs.emplace_back();
+        ch.emplace_back();
+        f2c.emplace_back();
+        cs.back().upload(tr.childStart, c.stream);
+        ch.back().upload(tr.child, c.stream);
+        f2c.back().upload(tr.fineToCoarse, c.stream);
+        c.sync();
+        if (!(fp32 && l == L - 1)) {
+            mats[l].reset(new DevMatrix<T>);
+            mats[l]->build(ag->levels[l].A, cl, c.stream);
+        }
+        work[l].ensure(ag->levels[l].nCoarse);
+    }
+    fw.ensure(fine->n);
+    fres.alloc(fine->n);
+    if (fp32) src32.alloc(fine->n);
+    pre.clear();
+    if (prm.nPre) {
+        pre.resize(L);
+        for (int l = 0; l < L; ++l) pre[l].alloc(ag->levels[l].nCoarse);
+    }
+    firstCall.assign(L, 1);
+    est.alloc(L);
+    mgst.alloc(1);
+
+    const HostLdu& CA = ag->levels[L - 1].A;
+    const Layout& CL = ag->layouts[L - 1];
+    if (!fp32) {
+        // reference path: PCG+DIC / PBiCGStab+DILU to relTol 1e-2 in one workgroup
+        coarsest64.reset();
+        DevMatrix<double>* cm = reinterpret_cast<DevMatrix<double>*>(mats[L - 1].get());
+        if (!cm->sweepable) throw Error(-3, "coarsest level faces are not owner-sorted");
+        if (CA.symm == false) cm->buildValU(CA, CL, c.stream);
+        std::vector<double> rdExt = CA.symm ? dicReciprocalD(CA) : diluReciprocalD(CA);
+        std::vector<double> rdInt(CL.n);
+        for (int i = 0; i < CL.n; ++i) rdInt[i] = rdExt[CL.perm[i]];
+        rDc.upload(rdInt, c.stream);
+        c.sync();
+        const int nvec = CA.symm ? 4 : 9;
+        coarseLds = sizeof(double) * ((size_t)nvec * CL.n + 64);
+        if (coarseLds > 160 * 1024)
+            throw Error(-4, "coarsest level too large for the one-workgroup solver (raise n_cells_coarsest limit)");
+        if (CA.symm)
+            UH_CHECK(hipFuncSetAttribute((const void*)kCoarsePCG,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)coarseLds));
+        else
+            UH_CHECK(hipFuncSetAttribute((const void*)kCoarseBiCG,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)coarseLds));
+    } else {
+        // mixed path: dense FP64 inverse of the coarsest Galerkin matrix
+        const int nc = CL.n;
+        if (nc > 8192) throw Error(-4, "coarsest level too large for the dense FP64 solve");
+        std::vector<double> D((size_t)nc * nc, 0.0);
+        for (int i = 0; i < nc; ++i) D[(size_t)CL.iperm[i] * nc + CL.iperm[i]] = CA.d[i];
+        const double* lw = CA.lower();
+        for (int f = 0; f < CA.nFaces(); ++f) {
+            const int a = CL.iperm[CA.l[f]], b = CL.iperm[CA.u[f]];
+            D[(size_t)a * nc + b] += CA.up[f];
+            D[(size_t)b * nc + a] += lw[f];
+        }
+        std::vector<double> inv = denseInverse(D, nc);
+        Ainv.upload(inv, c.stream);
+        c.sync();
+        // Chebyshev estimates once, here: x = 0 and a fixed pseudo-random b
+        if (prm.smoother == UNAP_HIP_SMOOTHER_CHEBY) {
+            for (int si = 0; si < L; ++si) {
+                DevMatrix<T>& M = (si == 0) ? *fine : *mats[si - 1];
+                LevelWork<T>& W = (si == 0) ? fw : work[si - 1];
+                std::vector<T> bh(M.n);
+                uint64_t s = 0x5eed + (uint64_t)si;
+                for (int i = 0; i < M.n; ++i) {
+                    s = s * 6364136223846793005ull + 1442695040888963407ull;
+                    bh[i] = (T)((double)(s >> 11) * 0x1.0p-53 * 2.0 - 1.0);
+                }
+                DBuf<T> bb;
+                bb.upload(bh, c.stream);
+                launchRows(BFill<T>{W.a.get(), (T)0}, M.n);
+                eigenEstimate<T>(M, W.a.get(), bb.get(), 10, est.get() + si, W.b.get(), W.c.get(),
+                                 W.d.get(), Skip{nullptr, nullptr});
+                c.sync();
+            }
+            for (auto& f : firstCall) f = 0;
+        }
+    }
+}
+
+template <typename T>
+void Hierarchy<T>::smooth(int si, T* x, const T* b, int nSweeps, Skip sk) {
+    DevMatrix<T>& M = (si == 0) ? *fine : *mats[si - 1];
+    LevelWork<T>& W = (si == 0) ? fw : work[si - 1];
+    if (prm.smoother == UNAP_HIP_SMOOTHER_GS) {
+        gsSmooth<T>(M, x, b, nSweeps, sk);
+        return;
+    }
+    // work vectors free while smoothing: finest a,b,c,d; coarse c,d,e
+    T* v1 = (si == 0) ? W.a.get() : W.c.get();
+    T* v2 = (si == 0) ? W.b.get() : W.d.get();
+    T* v3 = (si == 0) ? W.c.get() : W.e.get();
+    if (firstCall[si] && sizeof(T) == 8) {
+        eigenEstimate<T>(M, x, b, 10, est.get() + si, v1, v2, v3, sk);
+        firstCall[si] = 0;
+        return;
+    }
+    chebySweeps<T>(M, x, b, nSweeps, est.get() + si, v1, v2, sk);
+}
+
+template <typename T>
+void Hierarchy<T>::coarsestSolve(Skip sk) {
+    Ctx& c = Ctx::get();
+    const int l = L - 1;
+    T* corr = work[l].a.get();
+    const T* src = work[l].b.get();
+    const int nc = ag->levels[l].nCoarse;
+    if (sizeof(T) == 4) {
+        const int g = (nc + 3) / 4;
+        kDenseSolve<T><<<g, 256, 0, c.stream>>>(Ainv.get(), src, corr, nc, sk);
+        UH_LAUNCHED();
+        return;
+    }
+    DevMatrix<double>* cm = reinterpret_cast<DevMatrix<double>*>(mats[l].get());
+    const HostLdu& CA = ag->levels[l].A;
+    CoarseArgs a;
+    a.A = cm->view();
+    a.AP = CA.symm ? cm->viewU() : cm->view();
+    a.rD = rDc.get();
+    a.fwdStart = cm->fwdStart.get();
+    a.nFwd = (int)cm->fwdStartH.size() - 1;
+    a.bwdRows = cm->bwdRows.get();
+    a.bwdStart = cm->bwdStart.get();
+    a.nBwd = (int)cm->bwdStartH.size() - 1;
+    a.src = reinterpret_cast<const double*>(src);
+    a.corr = reinterpret_cast<double*>(corr);
+    a.relTol = 1.0e-2;
+    a.tol = 0.0;
+    a.maxIter = 100;
+    a.iperm = cm->iperm.get();
+    a.serial = c.serialReductions;
+    if (CA.symm)
+        kCoarsePCG<<<1, 1024, coarseLds, c.stream>>>(a, sk);
+    else
+        kCoarseBiCG<<<1, 1024, coarseLds, c.stream>>>(a, sk);
+    UH_LAUNCHED();
+}
+
+template <typename T>
+void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, const double* castFrom, Skip sk) {
+    const int coarsest = L - 1;
+    // restrict the finest residual (Vcycle.cpp:43-45)
+    if (castFrom)
+        launchRows(BRestrict<double, T>{cs[0].get(), ch[0].get(), castFrom, work[0].b.get(), fRes}, nl(0), sk);
+    else
+        launchRows(BRestrict<T, T>{cs[0].get(), ch[0].get(), fRes, work[0].b.get(), nullptr}, nl(0), sk);
+    for (int l = 0; l < coarsest; ++l) {
+        LevelWork<T>& W = work[l];
+        const int n = nl(l);
+        if (prm.nPre) {
+            launchRows(BFill<T>{W.a.get(), (T)0}, n, sk);
+            smooth(l + 1, W.a.get(), W.b.get(), prm.nPre + l, sk);
+            if (scale && l < coarsest - 1) {
+                launchReduce<2>(BSfAmul<T>{mat(l).view(), W.a.get(), W.b.get(), W.c.get()}, n,
+                                mat(l).iperm.get(), FinSF{mgst.get(), l}, sk);
+                launchRows(BJacobiScale<T>{W.a.get(), W.b.get(), W.c.get(), mat(l).diag.get(), mgst.get(), l,
+                                           nullptr},
+                           n, sk);
+            }
+            launchRows(BSubAmul<T>{mat(l).view(), W.a.get(), W.b.get()}, n, sk);
+        }
+        launchRows(BRestrict<T, T>{cs[l + 1].get(), ch[l + 1].get(), W.b.get(), work[l + 1].b.get(), nullptr},
+                   nl(l + 1), sk);
+    }
+    coarsestSolve(sk);
+    for (int l = coarsest - 1; l >= 0; --l) {
+        LevelWork<T>& W = work[l];
+        const int n = nl(l);
+        if (prm.nPre) launchRows(BCopy<T, T>{W.a.get(), pre[l].get()}, n, sk);
+        launchRows(BProlong<T>{f2c[l + 1].get(), work[l + 1].a.get(), W.a.get()}, n, sk);
+        if (scale && l < coarsest - 1) {
+            launchReduce<2>(BSfAmul<T>{mat(l).view(), W.a.get(), W.b.get(), W.c.get()}, n,
+                            mat(l).iperm.get(), FinSF{mgst.get(), l}, sk);
+            launchRows(BJacobiScale<T>{W.a.get(), W.b.get(), W.c.get(), mat(l).diag.get(), mgst.get(), l,
+                                       prm.nPre ? pre[l].get() : nullptr},
+                       n, sk);
+        } else if (prm.nPre) {
+            launchRows(BAddTo<T>{W.a.get(), pre[l].get()}, n, sk);
+        }
+        smooth(l + 1, W.a.get(), W.b.get(), prm.nPost + l, sk);
+    }
+    const int n = fine->n;
+    launchRows(BProlong<T>{f2c[0].get(), work[0].a.get(), fw.a.get()}, n, sk);
+    if (scale) {
+        launchReduce<2>(BSfAmul<T>{fine->view(), fw.a.get(), fRes, fw.b.get()}, n, fine->iperm.get(), FinSF{mgst.get(), 63}, sk);
+        launchRows(BPsiScaled<T>{psi, fw.a.get(), fRes, fw.b.get(), fine->diag.get(), mgst.get(), 63}, n, sk);
+    } else {
+        launchRows(BAddTo<T>{psi, fw.a.get()}, n, sk);
+    }
+    smooth(0, psi, source, prm.nFinest, sk);
+}
+
+// precondition(w, r): w = 0; nVcycles V-cycles of MGSolver::solve(w, A, r)
+// with tol = relTol = 0 (the residual after the last cycle is never used).
+template <typename T>
+void Hierarchy<T>::precondition(const double* r, T* w, Skip sk) {
+    const int n = fine->n;
+    launchRows(BFill<T>{w, (T)0}, n, sk);
+    const bool fp32 = sizeof(T) == 4;
+    T* source = fp32 ? src32.get() : reinterpret_cast<T*>(const_cast<double*>(r));
+    for (int cyc = 0; cyc < prm.nVcycles; ++cyc) {
+        if (cyc == 0) {
+            if (fp32)
+                vcycle(w, source, src32.get(), r, sk);
+            else
+                vcycle(w, source, source, nullptr, sk);
+        } else {
+            launchRows(BResidNoNorm<T>{fine->view(), w, source, fw.b.get(), fres.get()}, n, sk);
+            vcycle(w, source, fres.get(), nullptr, sk);
+        }
+    }
+}
+
+template struct Hierarchy<double>;
+template struct Hierarchy<float>;
+
+struct BDiagMul {
+    const double* rD;
+    const double* r;
+    double* w;
+    __device__ void operator()(int i) const { w[i] = rD[i] * r[i]; }
+};
+
+void applyPrecond64(PrecondObj& P, const double* r, double* w, Skip sk) {
+    const int n = P.A->n();
+    switch (P.type) {
+        case UNAP_HIP_PRECOND_NONE:
+            launchRows(BCopy<double, double>{r, w}, n, sk);
+            break;
+        case UNAP_HIP_PRECOND_DIAG:
+            launchRows(BDiagMul{P.rD.get(), r, w}, n, sk);
+            break;
+        case UNAP_HIP_PRECOND_DIC:
+        case UNAP_HIP_PRECOND_DILU:
+            luSweeps<double, double>(P.A->dev64(), P.type == UNAP_HIP_PRECOND_DIC, w, r, P.rD.get(), sk);
+            break;
+        case UNAP_HIP_PRECOND_AMG:
+            if (!P.amg64) throw Error(-4, "FP32 AMG preconditioner needs the FCG solver");
+            P.amg64->precondition(r, w, sk);
+            break;
+        default:
+            throw Error(-1, "unknown preconditioner");
+    }
+}
+
+// ------------------------------------------------------------- Krylov drivers
+struct SolveBufs {
+    int n = -1;
+    DBuf<double> x, b, v[8];
+    DBuf<float> z32[2];
+    DBuf<double> hist;
+    DBuf<KState> st;
+    void ensure(int nn, long long histCap, bool f32) {
+        if (nn != n) {
+            x.alloc(nn);
+            b.alloc(nn);
+            for (auto& q : v) q.alloc(nn);
+            n = nn;
+            z32[0].release();
+            z32[1].release();
+        }
+        if (f32 && !z32[0].p) {
+            z32[0].alloc(nn);
+            z32[1].alloc(nn);
+        }
+        if ((long long)hist.n < histCap) hist.alloc(histCap);
+        if (!st.p) st.alloc(1);
+    }
+};
+
+// Iterations to enqueue before the next flag read-back: the count that reaches
+// the tolerance at the observed mean convergence rate, between 1 and 16.
+// Iterations enqueued past convergence are skipped on the device.
+static int nextBatch(const KState& h, int cur) {
+    int b = std::min(cur * 2, 16);
+    if (h.nIter >= 1 && h.finalRes > 0 && h.initRes > 0 && h.relTol > 0 && h.finalRes < h.initRes) {
+        const double rate = std::log(h.finalRes / h.initRes) / (double)h.nIter;  // < 0
+        const double need = std::log(h.relTol * h.initRes / h.finalRes) / rate;
+        if (std::isfinite(need)) b = (int)std::ceil(std::max(1.0, need));
+    }
+    const long long left = h.maxIter - h.nIter;
+    if (left > 0 && left < b) b = (int)left;
+    return std::max(1, std::min(b, 16));
+}
+
+struct Driver {
+    SolverObj& S;
+    LduObj& A;
+    SolveBufs& B;
+    KState* st;
+    int n;
+    Driver(SolverObj& s, LduObj& a, SolveBufs& b) : S(s), A(a), B(b), st(b.st.get()), n(a.n()) {}
+
+    template <class Enqueue>
+    void iterate(Enqueue&& oneIteration) {
+        long long it = 0;
+        int batch = 1;
+        for (;;) {
+            for (int k = 0; k < batch; ++k) oneIteration(it++);
+            const KState& h = readState(st);
+            if (h.done) return;
+            batch = nextBatch(h, batch);
+        }
+    }
+
+    void pcg() {
+        double *x = B.x.get(), *b = B.b.get(), *r = B.v[0].get(), *w = B.v[1].get(), *p = B.v[2].get();
+        DevMatrix<double>& M = A.dev64();
+        launchReduce<1>(BResidual<double>{M.view(), x, b, r, nullptr}, n, M.iperm.get(), FinInitRes{st, 0});
+        const Skip sk{&st->done, nullptr};
+        PrecondObj* P = S.P;
+        iterate([&](long long) {
+            if (!P || P->type == UNAP_HIP_PRECOND_NONE)
+                launchReduce<1>(BCopyDot{r, w}, n, M.iperm.get(), FinPcgRho{st}, sk);
+            else if (P->type == UNAP_HIP_PRECOND_DIAG)
+                launchReduce<1>(BDiagDot{P->rD.get(), r, w}, n, M.iperm.get(), FinPcgRho{st}, sk);
+            else {
+                applyPrecond64(*P, r, w, sk);
+                launchReduce<1>(BDot{w, r}, n, M.iperm.get(), FinPcgRho{st}, sk);
+            }
+            launchRows(BPcgP{w, p, st}, n, sk);
+            launchReduce<1>(BAmulDot{M.view(), p, w}, n, M.iperm.get(), FinPcgAlpha{st}, sk);
+            launchReduce<1>(BPcgXR{x, r, p, w, st}, n, M.iperm.get(), FinResNorm{st}, sk);
+        });
+    }
+
+    void precondInto(const double* in, double* out, Skip sk) {
+        if (!S.P)
+            launchRows(BCopy<double, double>{in, out}, n, sk);
+        else
+            applyPrecond64(*S.P, in, out, sk);
+    }
+
+    void bicg() {
+        double *x = B.x.get(), *b = B.b.get();
+        double *r = B.v[0].get(), *r0 = B.v[1].get(), *p = B.v[2].get(), *y = B.v[3].get();
+        double *Ay = B.v[4].get(), *s = B.v[5].get(), *z = B.v[6].get(), *t = B.v[7].get();
+        DevMatrix<double>& M = A.dev64();
+        launchReduce<1>(BResidual<double>{M.view(), x, b, r, r0}, n, M.iperm.get(), FinInitRes{st, 0});
+        launchRows(BFill<double>{Ay, 0.0}, n);
+        launchRows(BFill<double>{p, 0.0}, n);
+        const Skip sk{&st->done, nullptr};
+        iterate([&](long long it) {
+            launchReduce<1>(BDot{r0, r}, n, M.iperm.get(), FinBicgRho{st}, sk);
+            launchRows(BBicgP{r, Ay, p, st}, n, sk);
+            precondInto(p, y, sk);
+            launchReduce<1>(BAmulDotR0{M.view(), y, r0, Ay}, n, M.iperm.get(), FinBicgAlpha{st}, sk);
+            launchReduce<1>(BBicgS{r, Ay, s, st}, n, M.iperm.get(), FinBicgS{st, it}, sk);
+            {
+                const Launch L = rowLaunch(n);
+                kBicgEarlyX<<<L.grid, kBlock, 0, Ctx::get().stream>>>(x, y, st, it, n);
+                UH_LAUNCHED();
+            }
+            precondInto(s, z, sk);
+            launchReduce<2>(BAmulTT{M.view(), z, s, t}, n, M.iperm.get(), FinBicgOmega{st}, sk);
+            launchReduce<1>(BBicgXR{x, r, y, z, s, t, st}, n, M.iperm.get(), FinResNorm{st}, sk);
+        });
+    }
+
+    template <typename TZ>
+    void fcgWith(TZ* z0, TZ* z1, std::function<void(const double*, TZ*, Skip)> prec) {
+        double *x = B.x.get(), *b = B.b.get(), *r = B.v[0].get(), *q = B.v[1].get(), *p = B.v[2].get();
+        DevMatrix<double>& M = A.dev64();
+        launchReduce<1>(BResidual<double>{M.view(), x, b, r, nullptr}, n, M.iperm.get(), FinInitRes{st, 0});
+        const Skip sk{&st->done, nullptr};
+        TZ* z = z0;
+        TZ* zo = z1;
+        iterate([&](long long it) {
+            prec(r, z, sk);
+            launchReduce<2>(BFcgRz<TZ>{r, z, zo, it == 0 ? 1 : 0}, n, M.iperm.get(), FinFcgRz{st}, sk);
+            launchRows(BFcgP<TZ>{z, p, st}, n, sk);
+            launchReduce<1>(BAmulDot{M.view(), p, q}, n, M.iperm.get(), FinPcgAlpha{st}, sk);
+            launchReduce<1>(BPcgXR{x, r, p, q, st}, n, M.iperm.get(), FinResNorm{st}, sk);
+            std::swap(z, zo);
+        });
+    }
+
+    void fcg() {
+        if (S.P && S.P->amg32) {
+            Hierarchy<float>* H = S.P->amg32.get();
+            fcgWith<float>(B.z32[0].get(), B.z32[1].get(),
+                           [H](const double* r, float* z, Skip sk) { H->precondition(r, z, sk); });
+        } else {
+            fcgWith<double>(B.v[3].get(), B.v[4].get(),
+                            [this](const double* r, double* z, Skip sk) { precondInto(r, z, sk); });
+        }
+    }
+
+    void mg(Hierarchy<double>& H) {
+        double *x = B.x.get(), *b = B.b.get(), *fres = B.v[0].get();
+        DevMatrix<double>& M = A.dev64();
+        H.initSmoothers();
+        launchReduce<1>(BResidual<double>{M.view(), x, b, fres, nullptr}, n, M.iperm.get(), FinInitRes{st, 1});
+        const Skip sk{&st->done, nullptr};
+        iterate([&](long long) {
+            H.vcycle(x, b, fres, nullptr, sk);
+            launchReduce<1>(BResidual<double>{M.view(), x, b, fres, nullptr}, n, M.iperm.get(), FinResNorm{st}, sk);
+        });
+    }
+};
+
+static std::shared_ptr<AgglHost> makeAggl(const HostLdu& A, const double* weights, bool normalise,
+                                          int maxLevels, int nCoarsest) {
+    std::vector<double> w(A.nFaces());
+    if (weights) {
+        for (int f = 0; f < A.nFaces(); ++f) w[f] = weights[f];
+    } else if (normalise) {
+        // |upper| / ||upper||_2 (fortranInterface.cpp:383-395)
+        double s = 0;
+        for (int f = 0; f < A.nFaces(); ++f) s += A.up[f] * A.up[f];
+        const double avg = std::sqrt(s);
+        for (int f = 0; f < A.nFaces(); ++f) w[f] = std::fabs(A.up[f]) / avg;
+    } else {
+        for (int f = 0; f < A.nFaces(); ++f) w[f] = std::fabs(A.up[f]);  // MGPreconditioner.cpp:14-19
+    }
+    auto ag = std::make_shared<AgglHost>();
+    buildAggl(A, w, maxLevels, nCoarsest, *ag);
+    return ag;
+}
+
+}  // namespace unap_hip
+
+// ====================================================================== C-ABI
+using namespace unap_hip;
+
+struct unap_hip_ldu_s {
+    LduObj o;
+};
+struct unap_hip_precond_s {
+    PrecondObj o;
+};
+struct unap_hip_smoother_s {
+    SmootherObj o{1};
+    LevelWork<double> w;
+};
+struct unap_hip_aggl_s {
+    LduObj* A;
+    std::shared_ptr<AgglHost> ag;
+    std::vector<DBuf<int>> cs, ch, f2c;
+};
+struct unap_hip_solver_s {
+    SolverObj o;
+    SolveBufs bufs;
+};
+
+namespace {
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        g_lastError.clear();
+        return UNAP_HIP_OK;
+    } catch (const Error& e) {
+        g_lastError = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_lastError = "host out of memory";
+        return UNAP_HIP_ERR_INTERNAL;
+    } catch (const std::exception& e) {
+        g_lastError = e.what();
+        return UNAP_HIP_ERR_INTERNAL;
+    }
+}
+
+void need(bool c, const char* msg) {
+    if (!c) throw Error(UNAP_HIP_ERR_ARG, msg);
+}
+
+// External-order vector -> internal-order device vector.
+void toInternal(const DevMatrix<double>& M, const double* src, int where, double* dst, DBuf<double>& stage) {
+    Ctx& c = Ctx::get();
+    const double* s = src;
+    if (where == UNAP_HIP_HOST) {
+        if ((int)stage.n < M.n) stage.alloc(M.n);
+        stage.upload(src, M.n, c.stream);
+        s = stage.get();
+    }
+    launchRows(BGather<double, double>{M.perm.get(), s, dst}, M.n);
+}
+void toExternal(const DevMatrix<double>& M, const double* src, int where, double* dst, DBuf<double>& stage) {
+    Ctx& c = Ctx::get();
+    if (where == UNAP_HIP_HOST) {
+        if ((int)stage.n < M.n) stage.alloc(M.n);
+        launchRows(BScatter<double, double>{M.perm.get(), src, stage.get()}, M.n);
+        UH_CHECK(hipMemcpyAsync(dst, stage.get(), sizeof(double) * M.n, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    } else {
+        launchRows(BScatter<double, double>{M.perm.get(), src, dst}, M.n);
+        c.sync();
+    }
+}
+
+DBuf<double>& stageBuf() {
+    static DBuf<double>* s = new DBuf<double>;
+    return *s;
+}
+
+void checkLdu(unap_hip_ldu A) { need(A != nullptr, "null matrix handle"); }
+
+}  // namespace
+
+extern "C" {
+
+const char* unap_hip_last_error(void) { return g_lastError.c_str(); }
+
+int unap_hip_set_reduction_order(int order) {
+    return guard([&] {
+        need(order == UNAP_HIP_REDUCE_TREE || order == UNAP_HIP_REDUCE_REFERENCE, "unknown reduction order");
+        Ctx::get().serialReductions = order;
+    });
+}
+
+int unap_hip_device_info(char* name, int64_t name_len, int64_t* cus) {
+    return guard([&] {
+        int dev = 0;
+        UH_CHECK(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        UH_CHECK(hipGetDeviceProperties(&p, dev));
+        if (name && name_len > 0) {
+            std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+        }
+        if (cus) *cus = p.multiProcessorCount;
+    });
+}
+
+int unap_hip_ldu_create(int64_t n, int64_t nf, const int64_t* la, const int64_t* ua, const double* lower,
+                        const double* diag, const double* upper, unap_hip_ldu* out) {
+    return guard([&] {
+        need(out != nullptr, "out is null");
+        need(n > 0 && n < (1LL << 31) && nf >= 0 && 2 * nf < (1LL << 31), "sizes out of range");
+        need(diag && (nf == 0 || (la && ua && upper)), "null array");
+        for (int64_t f = 0; f < nf; ++f)
+            need(la[f] >= 0 && la[f] < n && ua[f] >= 0 && ua[f] < n && la[f] != ua[f], "face address out of range");
+        auto* h = new unap_hip_ldu_s;
+        HostLdu& A = h->o.host;
+        A.n = (int)n;
+        A.l.assign(la, la + nf);
+        A.u.assign(ua, ua + nf);
+        A.d.assign(diag, diag + n);
+        A.up.assign(upper, upper + nf);
+        A.symm = (lower == nullptr || lower == upper);
+        if (!A.symm) A.lo.assign(lower, lower + nf);
+        try {
+            h->o.layout = buildLayout(A);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int unap_hip_ldu_destroy(unap_hip_ldu A) {
+    return guard([&] { delete A; });
+}
+
+int unap_hip_ldu_info(unap_hip_ldu A, int64_t* n, int64_t* nf, int* symm, int64_t* fl, int64_t* bl) {
+    return guard([&] {
+        checkLdu(A);
+        if (n) *n = A->o.host.n;
+        if (nf) *nf = A->o.host.nFaces();
+        if (symm) *symm = A->o.host.symm ? 1 : 0;
+        const bool sw = A->o.layout.sweepable;
+        if (fl) *fl = sw ? (int64_t)A->o.layout.fwdStart.size() - 1 : 0;
+        if (bl) *bl = sw ? (int64_t)A->o.layout.bwdStart.size() - 1 : 0;
+    });
+}
+
+int unap_hip_ldu_set_coeffs(unap_hip_ldu A, const double* diag, const double* upper, const double* lower) {
+    return guard([&] {
+        checkLdu(A);
+        HostLdu& H = A->o.host;
+        need(diag && (H.nFaces() == 0 || upper), "null array");
+        H.d.assign(diag, diag + H.n);
+        H.up.assign(upper, upper + H.nFaces());
+        H.symm = (lower == nullptr || lower == upper);
+        H.lo.clear();
+        if (!H.symm) H.lo.assign(lower, lower + H.nFaces());
+        Ctx& c = Ctx::get();
+        if (A->o.d64) A->o.d64->updateValues(H, A->o.layout, c.stream);
+        if (A->o.d32) A->o.d32->updateValues(H, A->o.layout, c.stream);
+    });
+}
+
+int unap_hip_ldu_spmv(unap_hip_ldu A, const double* psi, double* Apsi, int where) {
+    return guard([&] {
+        checkLdu(A);
+        need(psi && Apsi, "null vector");
+        DevMatrix<double>& M = A->o.dev64();
+        DBuf<double> xi(M.n), yi(M.n);
+        toInternal(M, psi, where, xi.get(), stageBuf());
+        launchRows(BAmul<double>{M.view(), xi.get(), yi.get()}, M.n);
+        toExternal(M, yi.get(), where, Apsi, stageBuf());
+    });
+}
+
+int unap_hip_amg_params_default(unap_hip_amg_params* p) {
+    return guard([&] {
+        need(p != nullptr, "null params");
+        p->smoother = UNAP_HIP_SMOOTHER_GS;
+        p->n_cells_coarsest = 50;
+        p->max_levels = 50;
+        p->n_vcycles = 2;
+        p->n_pre_sweeps = 0;
+        p->n_post_sweeps = 2;
+        p->n_finest_sweeps = 2;
+        p->precision = 64;
+    });
+}
+
+static AmgParams toParams(const unap_hip_amg_params* p) {
+    AmgParams a;
+    if (!p) return a;
+    a.smoother = (int)p->smoother;
+    a.nCoarsest = (int)p->n_cells_coarsest;
+    a.maxLevels = (int)p->max_levels;
+    a.nVcycles = (int)p->n_vcycles;
+    a.nPre = (int)p->n_pre_sweeps;
+    a.nPost = (int)p->n_post_sweeps;
+    a.nFinest = (int)p->n_finest_sweeps;
+    a.precision = (int)p->precision;
+    need(a.smoother == 1 || a.smoother == 2, "smoother must be GS (1) or Chebyshev (2)");
+    need(a.precision == 64 || a.precision == 32, "precision must be 64 or 32");
+    need(a.nVcycles >= 1 && a.nPre >= 0 && a.nPost >= 0 && a.nFinest >= 0, "bad sweep counts");
+    need(a.maxLevels >= 2 && a.maxLevels <= 60 && a.nCoarsest >= 1, "bad level limits");
+    return a;
+}
+
+int unap_hip_precond_create(unap_hip_ldu A, int type, const unap_hip_amg_params* amg, const double* fw,
+                            unap_hip_precond* out) {
+    return guard([&] {
+        checkLdu(A);
+        need(out != nullptr, "out is null");
+        need(type >= 0 && type <= 4, "unknown preconditioner type");
+        std::unique_ptr<unap_hip_precond_s> P(new unap_hip_precond_s);
+        PrecondObj& o = P->o;
+        o.type = type;
+        o.A = &A->o;
+        const HostLdu& H = A->o.host;
+        const Layout& L = A->o.layout;
+        Ctx& c = Ctx::get();
+        if (type == UNAP_HIP_PRECOND_DIAG || type == UNAP_HIP_PRECOND_DIC || type == UNAP_HIP_PRECOND_DILU) {
+            if (type == UNAP_HIP_PRECOND_DIAG) {
+                o.rDext.resize(H.n);
+                for (int i = 0; i < H.n; ++i) o.rDext[i] = 1.0 / H.d[i];
+            } else {
+                if (!L.sweepable)
+                    throw Error(UNAP_HIP_ERR_FACE_ORDER, "DIC/DILU need faces sorted by owner with owner < neighbour");
+                o.rDext = (type == UNAP_HIP_PRECOND_DIC) ? dicReciprocalD(H) : diluReciprocalD(H);
+            }
+            std::vector<double> rin(H.n);
+            for (int i = 0; i < H.n; ++i) rin[i] = o.rDext[L.perm[i]];
+            o.rD.upload(rin, c.stream);
+            if (type == UNAP_HIP_PRECOND_DIC) A->o.dev64().buildValU(H, L, c.stream);
+            c.sync();
+        } else if (type == UNAP_HIP_PRECOND_AMG) {
+            AmgParams prm = toParams(amg);
+            auto ag = makeAggl(H, fw, false, prm.maxLevels, prm.nCoarsest);
+            if (prm.precision == 32) {
+                o.amg32.reset(new Hierarchy<float>);
+                o.amg32->build(A->o, ag, prm);
+            } else {
+                o.amg64.reset(new Hierarchy<double>);
+                o.amg64->build(A->o, ag, prm);
+            }
+        }
+        *out = P.release();
+    });
+}
+
+int unap_hip_precond_destroy(unap_hip_precond P) {
+    return guard([&] { delete P; });
+}
+
+int unap_hip_precond_apply(unap_hip_precond P, const double* r, double* w, int where) {
+    return guard([&] {
+        need(P && r && w, "null argument");
+        PrecondObj& o = P->o;
+        DevMatrix<double>& M = o.A->dev64();
+        DBuf<double> ri(M.n), wi(M.n);
+        toInternal(M, r, where, ri.get(), stageBuf());
+        if (o.amg32) {
+            DBuf<float> w32(M.n);
+            o.amg32->initSmoothers();
+            o.amg32->precondition(ri.get(), w32.get(), Skip{nullptr, nullptr});
+            launchRows(BCopy<float, double>{w32.get(), wi.get()}, M.n);
+        } else {
+            if (o.amg64) o.amg64->initSmoothers();
+            applyPrecond64(o, ri.get(), wi.get(), Skip{nullptr, nullptr});
+        }
+        toExternal(M, wi.get(), where, w, stageBuf());
+    });
+}
+
+int unap_hip_precond_rd(unap_hip_precond P, double* rD) {
+    return guard([&] {
+        need(P && rD, "null argument");
+        if (P->o.rDext.empty()) throw Error(UNAP_HIP_ERR_STATE, "this preconditioner has no rD");
+        std::memcpy(rD, P->o.rDext.data(), sizeof(double) * P->o.rDext.size());
+    });
+}
+
+int unap_hip_smoother_create(int type, unap_hip_smoother* out) {
+    return guard([&] {
+        need(out != nullptr, "out is null");
+        need(type == UNAP_HIP_SMOOTHER_GS || type == UNAP_HIP_SMOOTHER_CHEBY, "unknown smoother");
+        auto* s = new unap_hip_smoother_s;
+        s->o.type = type;
+        *out = s;
+    });
+}
+
+int unap_hip_smoother_destroy(unap_hip_smoother S) {
+    return guard([&] { delete S; });
+}
+
+int unap_hip_smoother_init(unap_hip_smoother S) {
+    return guard([&] {
+        need(S != nullptr, "null smoother");
+        S->o.firstCall = true;
+    });
+}
+
+int unap_hip_smoother_smooth(unap_hip_smoother S, unap_hip_ldu A, double* x, const double* b, int64_t nSweeps,
+                             int where) {
+    return guard([&] {
+        need(S && A && x && b, "null argument");
+        need(nSweeps >= 0, "negative sweep count");
+        DevMatrix<double>& M = A->o.dev64();
+        DBuf<double> xi(M.n), bi(M.n);
+        toInternal(M, x, where, xi.get(), stageBuf());
+        toInternal(M, b, where, bi.get(), stageBuf());
+        S->w.ensure(M.n);
+        const Skip none{nullptr, nullptr};
+        if (S->o.type == UNAP_HIP_SMOOTHER_GS) {
+            gsSmooth<double>(M, xi.get(), bi.get(), (int)nSweeps, none);
+        } else if (S->o.firstCall) {
+            eigenEstimate<double>(M, xi.get(), bi.get(), 10, S->o.est.get(), S->w.a.get(), S->w.b.get(),
+                                  S->w.c.get(), none);
+            S->o.firstCall = false;
+        } else {
+            chebySweeps<double>(M, xi.get(), bi.get(), (int)nSweeps, S->o.est.get(), S->w.a.get(), S->w.b.get(),
+                                none);
+        }
+        toExternal(M, xi.get(), where, x, stageBuf());
+    });
+}
+
+int unap_hip_eigen_estimate(unap_hip_ldu A, double* x, const double* b, int64_t nPCGs, int where, double* lam) {
+    return guard([&] {
+        need(A && x && b && lam, "null argument");
+        DevMatrix<double>& M = A->o.dev64();
+        DBuf<double> xi(M.n), bi(M.n), p(M.n), w(M.n), r(M.n);
+        toInternal(M, x, where, xi.get(), stageBuf());
+        toInternal(M, b, where, bi.get(), stageBuf());
+        KState* est = Ctx::get().scratchState.get();
+        eigenEstimate<double>(M, xi.get(), bi.get(), (int)nPCGs, est, p.get(), w.get(), r.get(),
+                              Skip{nullptr, nullptr});
+        toExternal(M, xi.get(), where, x, stageBuf());
+        *lam = readState(est).lambda;
+    });
+}
+
+int unap_hip_aggl_create(unap_hip_ldu A, const double* weights, int64_t maxLevels, int64_t nCoarsest,
+                         unap_hip_aggl* out) {
+    return guard([&] {
+        need(A && weights && out, "null argument");
+        need(maxLevels >= 1 && maxLevels <= 60 && nCoarsest >= 1, "bad level limits");
+        auto* g = new unap_hip_aggl_s;
+        g->A = &A->o;
+        try {
+            g->ag = makeAggl(A->o.host, weights, false, (int)maxLevels, (int)nCoarsest);
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+
+int unap_hip_aggl_destroy(unap_hip_aggl G) {
+    return guard([&] { delete G; });
+}
+
+int unap_hip_aggl_nlevels(unap_hip_aggl G, int64_t* n) {
+    return guard([&] {
+        need(G && n, "null argument");
+        *n = (int64_t)G->ag->levels.size();
+    });
+}
+
+static const CoarseLevel& level(unap_hip_aggl G, int64_t l) {
+    need(G != nullptr, "null agglomeration");
+    need(l >= 0 && l < (int64_t)G->ag->levels.size(), "level out of range");
+    return G->ag->levels[l];
+}
+
+int unap_hip_aggl_level_size(unap_hip_aggl G, int64_t l, int64_t* nc, int64_t* nf) {
+    return guard([&] {
+        const CoarseLevel& C = level(G, l);
+        if (nc) *nc = C.nCoarse;
+        if (nf) *nf = C.A.nFaces();
+    });
+}
+
+int unap_hip_aggl_restrict_addressing(unap_hip_aggl G, int64_t l, int64_t* out) {
+    return guard([&] {
+        const CoarseLevel& C = level(G, l);
+        need(out != nullptr, "null output");
+        for (size_t i = 0; i < C.restrictMap.size(); ++i) out[i] = C.restrictMap[i];
+    });
+}
+
+int unap_hip_aggl_face_restrict_addressing(unap_hip_aggl G, int64_t l, int64_t* out) {
+    return guard([&] {
+        const CoarseLevel& C = level(G, l);
+        need(out != nullptr, "null output");
+        for (size_t i = 0; i < C.faceRestrictMap.size(); ++i) out[i] = C.faceRestrictMap[i];
+    });
+}
+
+int unap_hip_aggl_coarse_matrix(unap_hip_aggl G, int64_t l, int64_t* la, int64_t* ua, double* lower, double* diag,
+                                double* upper, int* symm) {
+    return guard([&] {
+        const CoarseLevel& C = level(G, l);
+        const HostLdu& A = C.A;
+        const int nf = A.nFaces();
+        for (int f = 0; f < nf; ++f) {
+            if (la) la[f] = A.l[f];
+            if (ua) ua[f] = A.u[f];
+            if (upper) upper[f] = A.up[f];
+            if (lower) lower[f] = A.lower()[f];
+        }
+        if (diag)
+            for (int i = 0; i < A.n; ++i) diag[i] = A.d[i];
+        if (symm) *symm = A.symm ? 1 : 0;
+    });
+}
+
+// restrict (fine level l -> coarse level l) / prolong (coarse l -> fine) on the
+// GPU through the same device transfer structures the V-cycle uses.
+static void ensureTransfers(unap_hip_aggl G) {
+    if (!G->cs.empty()) return;
+    Ctx& c = Ctx::get();
+    for (size_t l = 0; l < G->ag->levels.size(); ++l) {
+        const Layout& fl = (l == 0) ? G->A->layout : G->ag->layouts[l - 1];
+        Transfer tr = buildTransfer(fl, G->ag->layouts[l], G->ag->levels[l].restrictMap);
+        G->cs.emplace_back();
+        G->ch.emplace_back();
+        G->f2c.emplace_back();
+        G->cs.back().upload(tr.childStart, c.stream);
+        G->ch.back().upload(tr.child, c.stream);
+        G->f2c.back().upload(tr.fineToCoarse, c.stream);
+        c.sync();
+    }
+}
+
+int unap_hip_aggl_restrict_field(unap_hip_aggl G, int64_t l, const double* ff, double* cf, int where) {
+    return guard([&] {
+        level(G, l);
+        need(ff && cf, "null vector");
+        ensureTransfers(G);
+        Ctx& c = Ctx::get();
+        const Layout& fl = (l == 0) ? G->A->layout : G->ag->layouts[l - 1];
+        const Layout& cl = G->ag->layouts[l];
+        const int nf = fl.n, nc = cl.n;
+        DBuf<int> fperm, cperm;
+        fperm.upload(fl.perm, c.stream);
+        cperm.upload(cl.perm, c.stream);
+        DBuf<double> fi(nf), ci(nc), stage(std::max(nf, nc));
+        const double* src = ff;
+        if (where == UNAP_HIP_HOST) {
+            stage.upload(ff, nf, c.stream);
+            src = stage.get();
+        }
+        launchRows(BGather<double, double>{fperm.get(), src, fi.get()}, nf);
+        launchRows(BRestrict<double, double>{G->cs[l].get(), G->ch[l].get(), fi.get(), ci.get(), nullptr}, nc);
+        if (where == UNAP_HIP_HOST) {
+            launchRows(BScatter<double, double>{cperm.get(), ci.get(), stage.get()}, nc);
+            UH_CHECK(hipMemcpyAsync(cf, stage.get(), sizeof(double) * nc, hipMemcpyDeviceToHost, c.stream));
+        } else {
+            launchRows(BScatter<double, double>{cperm.get(), ci.get(), cf}, nc);
+        }
+        c.sync();
+    });
+}
+
+int unap_hip_aggl_prolong_field(unap_hip_aggl G, int64_t l, const double* cf, double* ff, int where) {
+    return guard([&] {
+        level(G, l);
+        need(ff && cf, "null vector");
+        ensureTransfers(G);
+        Ctx& c = Ctx::get();
+        const Layout& fl = (l == 0) ? G->A->layout : G->ag->layouts[l - 1];
+        const Layout& cl = G->ag->layouts[l];
+        const int nf = fl.n, nc = cl.n;
+        DBuf<int> fperm, cperm;
+        fperm.upload(fl.perm, c.stream);
+        cperm.upload(cl.perm, c.stream);
+        DBuf<double> fi(nf), ci(nc), stage(std::max(nf, nc));
+        const double* src = cf;
+        if (where == UNAP_HIP_HOST) {
+            stage.upload(cf, nc, c.stream);
+            src = stage.get();
+        }
+        launchRows(BGather<double, double>{cperm.get(), src, ci.get()}, nc);
+        launchRows(BProlong<double>{G->f2c[l].get(), ci.get(), fi.get()}, nf);
+        if (where == UNAP_HIP_HOST) {
+            launchRows(BScatter<double, double>{fperm.get(), fi.get(), stage.get()}, nf);
+            UH_CHECK(hipMemcpyAsync(ff, stage.get(), sizeof(double) * nf, hipMemcpyDeviceToHost, c.stream));
+        } else {
+            launchRows(BScatter<double, double>{fperm.get(), fi.get(), ff}, nf);
+        }
+        c.sync();
+    });
+}
+
+int unap_hip_solver_create(int type, unap_hip_solver* out) {
+    return guard([&] {
+        need(out != nullptr, "out is null");
+        need(type >= 1 && type <= 4, "unknown solver type");
+        auto* s = new unap_hip_solver_s;
+        s->o.type = type;
+        *out = s;
+    });
+}
+
+int unap_hip_solver_destroy(unap_hip_solver S) {
+    return guard([&] { delete S; });
+}
+
+int unap_hip_solver_set_precond(unap_hip_solver S, unap_hip_precond P) {
+    return guard([&] {
+        need(S != nullptr, "null solver");
+        S->o.P = P ? &P->o : nullptr;
+    });
+}
+
+int unap_hip_solver_set_limits(unap_hip_solver S, int64_t maxIter, int64_t minIter, double relTol, double tol) {
+    return guard([&] {
+        need(S != nullptr, "null solver");
+        need(maxIter >= 0 && minIter >= 0, "negative iteration limit");
+        S->o.maxIter = maxIter;
+        S->o.minIter = minIter;
+        S->o.relTol = relTol;
+        S->o.tol = tol;
+    });
+}
+
+int unap_hip_solver_set_mg(unap_hip_solver S, const unap_hip_amg_params* amg, const double* weights) {
+    return guard([&] {
+        need(S != nullptr, "null solver");
+        S->o.mg = toParams(amg);
+        need(S->o.mg.precision == 64, "the standalone MG solver runs in FP64");
+        S->o.mgSet = true;
+        S->o.mgH.reset();
+        S->o.mgFor = nullptr;
+        S->o.mgHasWeights = weights != nullptr;
+        S->o.mgWeights.clear();
+        S->o.mgWeightsPtr = weights;
+    });
+}
+
+int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const double* b, int where,
+                          unap_hip_perf* perf) {
+    return guard([&] {
+        need(S && A && x && b, "null argument");
+        SolverObj& o = S->o;
+        LduObj& L = A->o;
+        need(!o.P || o.P->A == &L, "preconditioner was built for another matrix");
+        need(!(o.P && o.P->amg32 && o.type != UNAP_HIP_SOLVER_FCG), "the FP32 AMG preconditioner needs FCG");
+        need(o.type != UNAP_HIP_SOLVER_MG || o.mgSet, "MG solver needs unap_hip_solver_set_mg first");
+        Ctx& c = Ctx::get();
+        if (where == UNAP_HIP_DEVICE) UH_CHECK(hipDeviceSynchronize());
+        DevMatrix<double>& M = L.dev64();
+        const long long histCap = o.maxIter + 2;
+        S->bufs.ensure(M.n, histCap, o.P && o.P->amg32);
+        if (o.type == UNAP_HIP_SOLVER_MG && (!o.mgH || o.mgFor != &L)) {
+            auto ag = makeAggl(L.host, o.mgWeightsPtr, true, o.mg.maxLevels, o.mg.nCoarsest);
+            o.mgH.reset(new Hierarchy<double>);
+            o.mgH->build(L, ag, o.mg);
+            o.mgFor = &L;
+        }
+        if (o.P && o.P->amg64) o.P->amg64->initSmoothers();
+        hipEvent_t e0, e1;
+        UH_CHECK(hipEventCreate(&e0));
+        UH_CHECK(hipEventCreate(&e1));
+        toInternal(M, x, where, S->bufs.x.get(), stageBuf());
+        toInternal(M, b, where, S->bufs.b.get(), stageBuf());
+        UH_CHECK(hipEventRecord(e0, c.stream));
+        KState* st = S->bufs.st.get();
+        kStateInit<<<1, 1, 0, c.stream>>>(st, o.tol, o.relTol, o.maxIter, o.minIter, S->bufs.hist.get(), histCap);
+        UH_LAUNCHED();
+        Driver d(o, L, S->bufs);
+        switch (o.type) {
+            case UNAP_HIP_SOLVER_PCG: d.pcg(); break;
+            case UNAP_HIP_SOLVER_PBICGSTAB: d.bicg(); break;
+            case UNAP_HIP_SOLVER_FCG: d.fcg(); break;
+            case UNAP_HIP_SOLVER_MG: d.mg(*o.mgH); break;
+        }
+        UH_CHECK(hipEventRecord(e1, c.stream));
+        toExternal(M, S->bufs.x.get(), where, x, stageBuf());
+        float ms = 0;
+        UH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        o.lastMs = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const KState& h = readState(st);
+        const long long nh = std::min<long long>(h.nIter + 1, histCap);
+        o.history.resize(nh);
+        UH_CHECK(hipMemcpy(o.history.data(), S->bufs.hist.get(), sizeof(double) * nh, hipMemcpyDeviceToHost));
+        if (perf) {
+            perf->initial_residual = h.initRes;
+            perf->final_residual = h.finalRes;
+            perf->previous_residual = h.prevRes;
+            perf->n_iterations = h.nIter;
+            perf->converged = h.converged;
+            perf->singular = h.singular;
+        }
+    });
+}
+
+int unap_hip_solver_history(unap_hip_solver S, double* out, int64_t cap, int64_t* len) {
+    return guard([&] {
+        need(S != nullptr, "null solver");
+        const auto& h = S->o.history;
+        if (len) *len = (int64_t)h.size();
+        if (out) std::memcpy(out, h.data(), sizeof(double) * std::min<size_t>(h.size(), (size_t)std::max<int64_t>(cap, 0)));
+    });
+}
+
+int unap_hip_solver_last_ms(unap_hip_solver S, double* ms) {
+    return guard([&] {
+        need(S && ms, "null argument");
+        *ms = S->o.lastMs;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,413 @@
+"""ctypes binding of the C-ABI in include/unap_hip.h (lib/libunap_hip.so).
+
+Mirrors the reference's C++ plug-in API: LduMatrix (lduMatrix), Precond
+(matrix::preconditioner), Smoother (matrix::smoother), Agglomeration
+(lduAgglomeration) and Solver (matrix::solver with solve(x, A, b)).  Vectors
+are numpy arrays (host) or objects exposing data_ptr() (device tensors).
+There is no CPU fallback: every call runs the HIP library.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libunap_hip.so")
+
+HOST, DEVICE = 0, 1
+PRECOND = {"none": 0, "diag": 1, "DIC": 2, "DILU": 3, "AMG": 4}
+SMOOTHER = {"GS": 1, "Cheby": 2}
+SOLVER = {"PCG": 1, "PBiCGStab": 2, "MG": 3, "FCG": 4}
+
+_i64 = ctypes.c_int64
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_intp = ctypes.POINTER(ctypes.c_int)
+
+
+class AmgParams(ctypes.Structure):
+    _fields_ = [("smoother", _i64), ("n_cells_coarsest", _i64), ("max_levels", _i64),
+                ("n_vcycles", _i64), ("n_pre_sweeps", _i64), ("n_post_sweeps", _i64),
+                ("n_finest_sweeps", _i64), ("precision", _i64)]
+
+
+class Perf(ctypes.Structure):
+    _fields_ = [("initial_residual", ctypes.c_double), ("final_residual", ctypes.c_double),
+                ("previous_residual", ctypes.c_double), ("n_iterations", _i64),
+                ("converged", ctypes.c_int32), ("singular", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {"initial": self.initial_residual, "final": self.final_residual,
+                "previous": self.previous_residual, "nIterations": int(self.n_iterations),
+                "converged": bool(self.converged), "singular": bool(self.singular)}
+
+
+# name -> (restype, argtypes); every function declared in include/unap_hip.h
+SIGNATURES = {
+    "unap_hip_last_error": (ctypes.c_char_p, []),
+    "unap_hip_device_info": (ctypes.c_int, [ctypes.c_char_p, _i64, _i64p]),
+    "unap_hip_set_reduction_order": (ctypes.c_int, [ctypes.c_int]),
+    "unap_hip_ldu_create": (ctypes.c_int, [_i64, _i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(_vp)]),
+    "unap_hip_ldu_destroy": (ctypes.c_int, [_vp]),
+    "unap_hip_ldu_info": (ctypes.c_int, [_vp, _i64p, _i64p, _intp, _i64p, _i64p]),
+    "unap_hip_ldu_set_coeffs": (ctypes.c_int, [_vp, _f64p, _f64p, _f64p]),
+    "unap_hip_ldu_spmv": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "unap_hip_amg_params_default": (ctypes.c_int, [ctypes.POINTER(AmgParams)]),
+    "unap_hip_precond_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(AmgParams), _f64p, ctypes.POINTER(_vp)]),
+    "unap_hip_precond_destroy": (ctypes.c_int, [_vp]),
+    "unap_hip_precond_apply": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "unap_hip_precond_rd": (ctypes.c_int, [_vp, _f64p]),
+    "unap_hip_smoother_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "unap_hip_smoother_destroy": (ctypes.c_int, [_vp]),
+    "unap_hip_smoother_smooth": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, ctypes.c_int]),
+    "unap_hip_smoother_init": (ctypes.c_int, [_vp]),
+    "unap_hip_eigen_estimate": (ctypes.c_int, [_vp, _vp, _vp, _i64, ctypes.c_int, _f64p]),
+    "unap_hip_aggl_create": (ctypes.c_int, [_vp, _f64p, _i64, _i64, ctypes.POINTER(_vp)]),
+    "unap_hip_aggl_destroy": (ctypes.c_int, [_vp]),
+    "unap_hip_aggl_nlevels": (ctypes.c_int, [_vp, _i64p]),
+    "unap_hip_aggl_level_size": (ctypes.c_int, [_vp, _i64, _i64p, _i64p]),
+    "unap_hip_aggl_restrict_addressing": (ctypes.c_int, [_vp, _i64, _i64p]),
+    "unap_hip_aggl_face_restrict_addressing": (ctypes.c_int, [_vp, _i64, _i64p]),
+    "unap_hip_aggl_coarse_matrix": (ctypes.c_int, [_vp, _i64, _i64p, _i64p, _f64p, _f64p, _f64p, _intp]),
+    "unap_hip_aggl_restrict_field": (ctypes.c_int, [_vp, _i64, _vp, _vp, ctypes.c_int]),
+    "unap_hip_aggl_prolong_field": (ctypes.c_int, [_vp, _i64, _vp, _vp, ctypes.c_int]),
+    "unap_hip_solver_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "unap_hip_solver_destroy": (ctypes.c_int, [_vp]),
+    "unap_hip_solver_set_precond": (ctypes.c_int, [_vp, _vp]),
+    "unap_hip_solver_set_limits": (ctypes.c_int, [_vp, _i64, _i64, ctypes.c_double, ctypes.c_double]),
+    "unap_hip_solver_set_mg": (ctypes.c_int, [_vp, ctypes.POINTER(AmgParams), _f64p]),
+    "unap_hip_solver_solve": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(Perf)]),
+    "unap_hip_solver_history": (ctypes.c_int, [_vp, _f64p, _i64, _i64p]),
+    "unap_hip_solver_last_ms": (ctypes.c_int, [_vp, _f64p]),
+    "unap_hip_batch_create": (ctypes.c_int, [_i64, _i64p, _i64p, _i64p, _i64p, _f64p, _f64p, _f64p, _i32p,
+                                             ctypes.POINTER(_vp)]),
+    "unap_hip_batch_destroy": (ctypes.c_int, [_vp]),
+    "unap_hip_batch_solve": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _i64, ctypes.c_double, ctypes.c_double,
+                                            ctypes.POINTER(Perf)]),
+    "unap_hip_batch_last_ms": (ctypes.c_int, [_vp, _f64p]),
+}
+
+_lib = None
+
+
+class UnapHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"unap_hip error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """The loaded C-ABI library.  Raises when it is missing: there is no fallback."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise UnapHipError(rc, lib().unap_hip_last_error().decode())
+
+
+def _p(a, t=_f64p):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _vec(v):
+    """(pointer, where, keepalive) for a numpy array or a device tensor."""
+    if hasattr(v, "data_ptr"):
+        return ctypes.c_void_p(v.data_ptr()), DEVICE, v
+    a = np.ascontiguousarray(v, dtype=np.float64)
+    return a.ctypes.data_as(ctypes.c_void_p), HOST, a
+
+
+REDUCE_TREE, REDUCE_REFERENCE = 0, 1
+
+
+def set_reduction_order(order):
+    """REDUCE_TREE (fast, default) or REDUCE_REFERENCE (the reference's serial sums)."""
+    _check(lib().unap_hip_set_reduction_order(order))
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(256)
+    cus = _i64()
+    _check(lib().unap_hip_device_info(buf, 256, ctypes.byref(cus)))
+    return buf.value.decode(), int(cus.value)
+
+
+def amg_params(**kw):
+    p = AmgParams()
+    _check(lib().unap_hip_amg_params_default(ctypes.byref(p)))
+    for k, v in kw.items():
+        setattr(p, k, int(v))
+    return p
+
+
+class LduMatrix:
+    """lduMatrix(nCells, lowerAddr, upperAddr, lower, diag, upper) on the GPU."""
+
+    def __init__(self, n, lower_addr, upper_addr, lower, diag, upper):
+        L = lib()
+        self._l = np.ascontiguousarray(lower_addr, np.int64)
+        self._u = np.ascontiguousarray(upper_addr, np.int64)
+        self._up = np.ascontiguousarray(upper, np.float64)
+        self._lo = None if lower is None or lower is upper else np.ascontiguousarray(lower, np.float64)
+        self._d = np.ascontiguousarray(diag, np.float64)
+        self.n = int(n)
+        h = _vp()
+        _check(L.unap_hip_ldu_create(self.n, self._u.shape[0], _p(self._l, _i64p), _p(self._u, _i64p),
+                                     _p(self._lo) if self._lo is not None else _p(self._up),
+                                     _p(self._d), _p(self._up), ctypes.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_system(cls, s):
+        return cls(s.n, s.l, s.u, None if s.symmetric else s.lower, s.diag, s.upper)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.unap_hip_ldu_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        n, nf, fl, bl = _i64(), _i64(), _i64(), _i64()
+        sy = ctypes.c_int()
+        _check(lib().unap_hip_ldu_info(self.h, ctypes.byref(n), ctypes.byref(nf), ctypes.byref(sy),
+                                       ctypes.byref(fl), ctypes.byref(bl)))
+        return {"n": n.value, "nfaces": nf.value, "symmetric": bool(sy.value),
+                "fwd_levels": fl.value, "bwd_levels": bl.value}
+
+    def spmv(self, x, out=None):
+        xp, where, keep = _vec(x)
+        if where == HOST:
+            y = np.empty(self.n)
+            _check(lib().unap_hip_ldu_spmv(self.h, xp, y.ctypes.data_as(ctypes.c_void_p), HOST))
+            return y
+        _check(lib().unap_hip_ldu_spmv(self.h, xp, ctypes.c_void_p(out.data_ptr()), DEVICE))
+        return out
+
+
+class Precond:
+    def __init__(self, A, kind, amg=None, weights=None):
+        self.A = A
+        h = _vp()
+        w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+        _check(lib().unap_hip_precond_create(A.h, PRECOND[kind], None if amg is None else ctypes.byref(amg),
+                                             _p(w), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.unap_hip_precond_destroy(self.h)
+            self.h = None
+
+    def apply(self, r):
+        rp, where, keep = _vec(r)
+        w = np.empty(self.A.n)
+        _check(lib().unap_hip_precond_apply(self.h, rp, w.ctypes.data_as(ctypes.c_void_p), HOST))
+        return w
+
+    def rD(self):
+        out = np.empty(self.A.n)
+        _check(lib().unap_hip_precond_rd(self.h, _p(out)))
+        return out
+
+
+class Smoother:
+    def __init__(self, kind):
+        h = _vp()
+        _check(lib().unap_hip_smoother_create(SMOOTHER[kind], ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.unap_hip_smoother_destroy(self.h)
+            self.h = None
+
+    def smooth(self, A, x, b, nsweeps):
+        x = np.array(x, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        _check(lib().unap_hip_smoother_smooth(self.h, A.h, x.ctypes.data_as(ctypes.c_void_p),
+                                              b.ctypes.data_as(ctypes.c_void_p), nsweeps, HOST))
+        return x
+
+    def init(self):
+        _check(lib().unap_hip_smoother_init(self.h))
+
+
+def eigen_estimate(A, x, b, npcg=10):
+    x = np.array(x, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    lam = ctypes.c_double()
+    _check(lib().unap_hip_eigen_estimate(A.h, x.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p),
+                                         npcg, HOST, ctypes.byref(lam)))
+    return lam.value, x
+
+
+class Agglomeration:
+    def __init__(self, A, weights, max_levels=50, n_coarsest=50):
+        self.A = A
+        w = np.ascontiguousarray(weights, np.float64)
+        h = _vp()
+        _check(lib().unap_hip_aggl_create(A.h, _p(w), max_levels, n_coarsest, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.unap_hip_aggl_destroy(self.h)
+            self.h = None
+
+    def nlevels(self):
+        n = _i64()
+        _check(lib().unap_hip_aggl_nlevels(self.h, ctypes.byref(n)))
+        return n.value
+
+    def level(self, lev, fine_n, fine_nf):
+        nc, nf = _i64(), _i64()
+        L = lib()
+        _check(L.unap_hip_aggl_level_size(self.h, lev, ctypes.byref(nc), ctypes.byref(nf)))
+        nc, nf = nc.value, nf.value
+        ra = np.empty(fine_n, np.int64)
+        fa = np.empty(fine_nf, np.int64)
+        _check(L.unap_hip_aggl_restrict_addressing(self.h, lev, _p(ra, _i64p)))
+        _check(L.unap_hip_aggl_face_restrict_addressing(self.h, lev, _p(fa, _i64p)))
+        l = np.empty(nf, np.int64)
+        u = np.empty(nf, np.int64)
+        lo = np.empty(nf)
+        up = np.empty(nf)
+        d = np.empty(nc)
+        sy = ctypes.c_int()
+        _check(L.unap_hip_aggl_coarse_matrix(self.h, lev, _p(l, _i64p), _p(u, _i64p), _p(lo), _p(d), _p(up),
+                                             ctypes.byref(sy)))
+        return {"n": nc, "nfaces": nf, "restrict": ra, "face_restrict": fa, "l": l, "u": u, "lower": lo,
+                "diag": d, "upper": up, "symmetric": bool(sy.value)}
+
+    def levels(self):
+        out = []
+        n, nf = self.A.n, self.A._u.shape[0]
+        for lev in range(self.nlevels()):
+            lv = self.level(lev, n, nf)
+            out.append(lv)
+            n, nf = lv["n"], lv["nfaces"]
+        return out
+
+    def restrict_field(self, lev, ff, nc):
+        ff = np.ascontiguousarray(ff, np.float64)
+        cf = np.empty(nc)
+        _check(lib().unap_hip_aggl_restrict_field(self.h, lev, ff.ctypes.data_as(ctypes.c_void_p),
+                                                  cf.ctypes.data_as(ctypes.c_void_p), HOST))
+        return cf
+
+    def prolong_field(self, lev, cf, nf):
+        cf = np.ascontiguousarray(cf, np.float64)
+        ff = np.empty(nf)
+        _check(lib().unap_hip_aggl_prolong_field(self.h, lev, cf.ctypes.data_as(ctypes.c_void_p),
+                                                 ff.ctypes.data_as(ctypes.c_void_p), HOST))
+        return ff
+
+
+class Solver:
+    """matrix::solver: PCG / PBiCGStab / MG / FCG with solve(x, A, b)."""
+
+    def __init__(self, kind, precond=None, max_iter=1000, min_iter=0, rel_tol=1e-6, tol=0.0, mg=None,
+                 mg_weights=None):
+        L = lib()
+        h = _vp()
+        _check(L.unap_hip_solver_create(SOLVER[kind], ctypes.byref(h)))
+        self.h = h
+        self.precond = precond
+        if precond is not None:
+            _check(L.unap_hip_solver_set_precond(h, precond.h))
+        _check(L.unap_hip_solver_set_limits(h, max_iter, min_iter, rel_tol, tol))
+        self._mgw = None if mg_weights is None else np.ascontiguousarray(mg_weights, np.float64)
+        if mg is not None:
+            _check(L.unap_hip_solver_set_mg(h, ctypes.byref(mg), _p(self._mgw)))
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.unap_hip_solver_destroy(self.h)
+            self.h = None
+
+    def solve(self, A, b, x0=None, x=None):
+        """Host arrays: returns (x, perf).  Device tensors: pass x (in/out) and b."""
+        perf = Perf()
+        if hasattr(b, "data_ptr"):
+            _check(lib().unap_hip_solver_solve(self.h, A.h, ctypes.c_void_p(x.data_ptr()),
+                                               ctypes.c_void_p(b.data_ptr()), DEVICE, ctypes.byref(perf)))
+            return x, perf.as_dict()
+        xx = np.zeros(A.n) if x0 is None else np.array(x0, np.float64)
+        bb = np.ascontiguousarray(b, np.float64)
+        _check(lib().unap_hip_solver_solve(self.h, A.h, xx.ctypes.data_as(ctypes.c_void_p),
+                                           bb.ctypes.data_as(ctypes.c_void_p), HOST, ctypes.byref(perf)))
+        d = perf.as_dict()
+        d["history"] = self.history()
+        return xx, d
+
+    def history(self):
+        n = _i64()
+        _check(lib().unap_hip_solver_history(self.h, None, 0, ctypes.byref(n)))
+        out = np.empty(n.value)
+        _check(lib().unap_hip_solver_history(self.h, _p(out), n.value, ctypes.byref(n)))
+        return out
+
+    def last_ms(self):
+        v = ctypes.c_double()
+        _check(lib().unap_hip_solver_last_ms(self.h, ctypes.byref(v)))
+        return v.value
+
+
+class Batch:
+    """Many independent LDU systems solved in one persistent kernel launch."""
+
+    def __init__(self, systems, solvers):
+        ns = len(systems)
+        cell_off = np.zeros(ns + 1, np.int64)
+        face_off = np.zeros(ns + 1, np.int64)
+        for i, s in enumerate(systems):
+            cell_off[i + 1] = cell_off[i] + s.n
+            face_off[i + 1] = face_off[i] + s.nfaces
+        self.l = np.concatenate([s.l for s in systems]).astype(np.int64)
+        self.u = np.concatenate([s.u for s in systems]).astype(np.int64)
+        self.up = np.concatenate([s.upper for s in systems])
+        self.lo = np.concatenate([s.lower_or_upper() for s in systems])
+        self.d = np.concatenate([s.diag for s in systems])
+        self.cell_off, self.face_off = cell_off, face_off
+        self.kinds = np.array([SOLVER[k] for k in solvers], np.int32)
+        h = _vp()
+        _check(lib().unap_hip_batch_create(ns, _p(cell_off, _i64p), _p(face_off, _i64p), _p(self.l, _i64p),
+                                           _p(self.u, _i64p), _p(self.lo), _p(self.d), _p(self.up),
+                                           _p(self.kinds, _i32p), ctypes.byref(h)))
+        self.h = h
+        self.ns = ns
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.unap_hip_batch_destroy(self.h)
+            self.h = None
+
+    def solve(self, b, x=None, max_iter=1000, rel_tol=1e-8, tol=0.0):
+        perfs = (Perf * self.ns)()
+        if hasattr(b, "data_ptr"):
+            _check(lib().unap_hip_batch_solve(self.h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+                                              DEVICE, max_iter, rel_tol, tol, perfs))
+            return x, [p.as_dict() for p in perfs]
+        xx = np.zeros(int(self.cell_off[-1])) if x is None else np.array(x, np.float64)
+        bb = np.ascontiguousarray(b, np.float64)
+        _check(lib().unap_hip_batch_solve(self.h, xx.ctypes.data_as(ctypes.c_void_p),
+                                          bb.ctypes.data_as(ctypes.c_void_p), HOST, max_iter, rel_tol, tol, perfs))
+        return xx, [p.as_dict() for p in perfs]
+
+    def last_ms(self):
+        v = ctypes.c_double()
+        _check(lib().unap_hip_batch_last_ms(self.h, ctypes.byref(v)))
+        return v.value
