@@ -1,0 +1,209 @@
+"""Benchmark of the solve path on one MI355X.
+
+Workload (C4): a 256^3 (16.8M cells, 50.1M faces) randomly relabelled
+variable-coefficient Poisson system in LDU form, face coefficients
+log-uniform in [1e-4, 1], solved by AMG-preconditioned CG to relative
+residual 1e-8 from x = 0.
+
+  * mixed (the headline `value`): flexible CG in FP64 (true residual,
+    dot products and norms in FP64) around one V-cycle of the pairwise
+    agglomeration AMG in FP32 (hierarchy, Chebyshev(4) smoothers on every
+    level, restriction and prolongation in FP32; coarsest level solved in
+    FP64 by a dense inverse);
+  * fp64 drop-in: the reference's algorithm in FP64 on the GPU (PCG around
+    MGSolver V-cycles with Chebyshev smoothers, nPost = nFinest = 4 with the
+    reference's nPost + level schedule, coarsest level by PCG+DIC to 1e-2).
+
+Inputs are resident in HBM before the timed region; each step is one full
+solve.  Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "AMG-PCG time to rel.res 1e-8 (256^3), mixed vs FP64; HBM GB/s"
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--n-coarsest", type=int, default=500)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-size", type=int, default=None, help="edge length of the CPU sample (default: --size)")
+    ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch of the dominant kernel")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
+              "described in DESIGN.md and not run here.", file=sys.stderr)
+        sys.exit(2)
+
+    from unap_amd import hip, meshgen
+
+    dev_name, cus = hip.device_info()
+    log("device:", dev_name, cus, "CUs")
+    N = args.size
+    t0 = time.time()
+    s = meshgen.config_system(4, N)
+    t_gen = time.time() - t0
+    log(f"generated {N}^3: n={s.n} faces={s.nfaces} in {t_gen:.1f}s")
+
+    t0 = time.time()
+    A = hip.LduMatrix.from_system(s)
+    t_layout = time.time() - t0
+    info = A.info()
+    log(f"layout {t_layout:.1f}s", info)
+
+    common = dict(smoother=2, n_cells_coarsest=args.n_coarsest, n_vcycles=1, n_post_sweeps=4, n_finest_sweeps=4)
+    t0 = time.time()
+    P32 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, level_sweep_growth=0, **common))
+    t_setup32 = time.time() - t0
+    log(f"mixed hierarchy setup {t_setup32:.1f}s (includes agglomeration)")
+    t0 = time.time()
+    P64 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=64, level_sweep_growth=1, **common))
+    t_setup64 = time.time() - t0
+    log(f"fp64 hierarchy setup {t_setup64:.1f}s")
+
+    b = hip.DeviceArray.from_numpy(s.b)
+    x = hip.DeviceArray(s.n)
+    S32 = hip.Solver("FCG", P32, rel_tol=1e-8, max_iter=1000)
+    S64 = hip.Solver("PCG", P64, rel_tol=1e-8, max_iter=1000)
+
+    def solve(S):
+        x.zero()
+        hip.synchronize()
+        t = time.perf_counter()
+        _, perf = S.solve(A, b, x=x)
+        hip.synchronize()
+        return time.perf_counter() - t, perf
+
+    for _ in range(args.warmup):
+        solve(S32)
+    log("warmup done")
+    times32 = []
+    for k in range(args.steps):
+        dt, p32 = solve(S32)
+        times32.append(dt)
+        log(f"mixed step {k}: {dt * 1e3:.1f} ms, {p32['nIterations']} iterations, converged={p32['converged']}")
+    x32 = x.numpy()
+    ms_dev32 = S32.last_ms()
+
+    times64 = []
+    for k in range(max(1, args.fp64_steps)):
+        dt, p64 = solve(S64)
+        times64.append(dt)
+        log(f"fp64 step {k}: {dt * 1e3:.1f} ms, {p64['nIterations']} iterations, converged={p64['converged']}")
+    x64 = x.numpy()
+    t64 = float(np.median(times64))
+
+    # accuracy: against an FP64 solve to 1e-12 (flexible CG around the FP64
+    # hierarchy: the reference's V-cycle is not a fixed linear operator, so
+    # plain PCG stalls before 1e-12), and the true residual
+    Sref = hip.Solver("FCG", P64, rel_tol=1e-12, max_iter=2000)
+    _, pref = solve(Sref)
+    xs = x.numpy()
+    err32 = float(np.linalg.norm(x32 - xs) / np.linalg.norm(xs))
+    err64 = float(np.linalg.norm(x64 - xs) / np.linalg.norm(xs))
+    r32 = s.b - A.spmv(x32)
+    true_res32 = float(np.linalg.norm(r32) / np.linalg.norm(s.b))
+    log(f"errors vs FP64@1e-12 ({pref['nIterations']} it): mixed {err32:.2e}, fp64@1e-8 {err64:.2e}; "
+        f"mixed true rel. residual {true_res32:.2e}")
+
+    # per-kernel rooflines on the finest level (HIP events on the library stream)
+    kernels = {}
+    for name, prec in [("cheby", 32), ("cheby", 64), ("amul", 32), ("amul", 64), ("residual", 64),
+                       ("axpy_norm", 64), ("dot", 64), ("copy", 64), ("copy", 32)]:
+        ms, by = hip.profile_kernel(A, name, prec, reps=20)
+        gbs = by / (ms * 1e-3) / 1e9
+        kernels[f"{name}_fp{prec}"] = {"ms": round(ms, 4), "alg_bytes": by, "GBps": round(gbs, 1),
+                                       "frac": round(gbs / PEAK_HBM_GBS, 3)}
+        log(f"kernel {name} fp{prec}: {ms:.3f} ms, {gbs:.0f} GB/s ({gbs / PEAK_HBM_GBS:.0%} of 8 TB/s)")
+    dom = kernels["cheby_fp32"]
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        traffic = json.load(open(args.traffic)).get("cheby_fp32_bytes")
+    roofline = {"kernel": "Chebyshev sweep, FP32, finest level (fused Amul + update)", "bound": "hbm",
+                "achieved": dom["GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": dom["frac"],
+                "alg_bytes_per_launch": dom["alg_bytes"], "traffic": traffic}
+
+    # CPU baseline: the reference's own x86 solve path (compiled from its
+    # sources into oracle/_ref) or, where that is absent, the oracle restatement.
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_size or N, p64["nIterations"], args.n_coarsest)
+
+    t32 = float(np.median(times32))
+    out = {
+        "metric": METRIC, "value": round(t32, 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t32 * 1e3, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"C4: {N}^3 permuted log-uniform(1e-4..1) Poisson LDU, AMG-PCG to rel.res 1e-8",
+                   "n_cells": s.n, "n_faces": s.nfaces, "mixed": "FCG(FP64) + FP32 AMG V(0,4) Chebyshev(4), "
+                   "coarsest dense FP64", "fp64": "PCG + FP64 AMG V(0,4+level) Chebyshev(4), coarsest PCG+DIC 1e-2",
+                   "n_cells_coarsest": args.n_coarsest},
+        "mixed": {"s": round(t32, 5), "device_ms": round(ms_dev32, 3), "iterations": p32["nIterations"],
+                  "converged": p32["converged"], "err_vs_fp64_1e-12": err32, "true_rel_residual": true_res32},
+        "fp64_drop_in": {"s": round(t64, 5), "iterations": p64["nIterations"], "converged": p64["converged"],
+                         "err_vs_fp64_1e-12": err64},
+        "speedup_mixed_vs_fp64": round(t64 / t32, 3),
+        "setup_s": {"generate": round(t_gen, 2), "layout": round(t_layout, 2), "mixed_hierarchy": round(t_setup32, 2),
+                    "fp64_hierarchy": round(t_setup64, 2)},
+        "device": dev_name, "roofline": roofline, "kernels": kernels, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(N, iters_fp64, n_coarsest):
+    """UNAP's x86 path timed on this host: a bounded sample of the FP64
+    drop-in's algorithm, extrapolated to the GPU FP64 iteration count."""
+    from oracle import oracle as O
+    from unap_amd import meshgen
+
+    path = O.reference()
+    kind = "reference"
+    if path is None:
+        path = O.oracle()
+        kind = "port"
+    s = meshgen.config_system(4, N)
+    M = path.matrix(s)
+    log(f"cpu baseline ({kind}) at {N}^3 ...")
+    perf, t_setup, t_cold, t_warm = O.timed_solve(path, M, "PCG", "AMG", s.b, 1, 1, 1e-8,
+                                                  mg=[2, n_coarsest, 1, 0, 4, 4])
+    est = t_cold + (iters_fp64 - 1) * t_warm
+    log(f"cpu: setup {t_setup:.1f}s, first iteration {t_cold:.1f}s, next iteration {t_warm:.1f}s -> "
+        f"{est:.1f}s for {iters_fp64} iterations")
+    return {"value": round(est, 2), "unit": "s", "cores": 1, "kind": kind,
+            "sample": (f"{N}^3 C4 system, FP64 PCG + AMG(Chebyshev 4, nPost+level) as the reference runs it: "
+                       f"one cold iteration ({t_cold:.2f}s, includes the eigen estimates) and one warm iteration "
+                       f"({t_warm:.2f}s) timed, extrapolated to the {iters_fp64} iterations of the GPU FP64 "
+                       f"solve; setup {t_setup:.1f}s excluded"),
+            "host_cpu": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip() + f" (nproc {os.cpu_count()})"
+    except OSError:
+        pass
+    return None
+
+
+if __name__ == "__main__":
+    main()

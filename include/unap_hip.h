@@ -110,6 +110,8 @@ typedef struct {
     int64_t n_post_sweeps;
     int64_t n_finest_sweeps;
     int64_t precision;              /* 64 or 32 */
+    int64_t level_sweep_growth;     /* 1: coarse level l smooths nPost + l times (Vcycle.cpp:250,
+                                       default); 0: every coarse level smooths nPost times */
 } unap_hip_amg_params;
 
 int unap_hip_amg_params_default(unap_hip_amg_params* p);
@@ -197,6 +199,23 @@ int unap_hip_solver_history(unap_hip_solver S, double* out, int64_t cap, int64_t
 /* Device time of the last solve in milliseconds (HIP events around the solve). */
 int unap_hip_solver_last_ms(unap_hip_solver S, double* ms);
 
+/* ------------------------------------------------------------- measurement
+ * No reference counterpart.  Times `reps` launches of one kernel of the solve
+ * path on the library's stream with HIP events, on matrix A's finest level,
+ * and returns the mean duration and the kernel's algorithmic HBM bytes per
+ * launch (each distinct array read or written counted once; padding, cache
+ * re-reads and gathers not counted).  Kernels: */
+#define UNAP_HIP_KERNEL_AMUL 1       /* y = A x (lduMatrixSpMV.cpp:16-98) */
+#define UNAP_HIP_KERNEL_CHEBY 2      /* one fused Chebyshev sweep (chebySmoother.cpp:105-144) */
+#define UNAP_HIP_KERNEL_GS 3         /* one Gauss-Seidel sweep, all levels */
+#define UNAP_HIP_KERNEL_DIC 4        /* DIC apply: forward + backward schedule */
+#define UNAP_HIP_KERNEL_RESIDUAL 5   /* r = b - A x and sum r^2 (+ finalize) */
+#define UNAP_HIP_KERNEL_AXPY_NORM 6  /* x += a p; r -= a w; sum r^2 (+ finalize) */
+#define UNAP_HIP_KERNEL_DOT 7        /* sum a b (+ finalize) */
+#define UNAP_HIP_KERNEL_COPY 8       /* y = x, a plain copy (achievable-bandwidth yardstick) */
+int unap_hip_profile_kernel(unap_hip_ldu A, int kernel, int precision, int64_t reps, double* avg_ms,
+                            double* bytes);
+
 /* ------------------------------------------------------------------ batches
  * Many independent LDU systems solved in one persistent kernel launch: one
  * workgroup runs one system's whole Krylov solve, systems are taken from a
@@ -214,7 +233,10 @@ int unap_hip_batch_create(int64_t n_systems, const int64_t* cell_off, const int6
 int unap_hip_batch_destroy(unap_hip_batch B);
 int unap_hip_batch_solve(unap_hip_batch B, double* x, const double* b, int where, int64_t max_iter,
                          double rel_tol, double tol, unap_hip_perf* perf);
+/* Device time of the last batch launch (HIP events), and the launch shape the
+   autotuner chose on the first solve (threads per workgroup, workgroups per CU). */
 int unap_hip_batch_last_ms(unap_hip_batch B, double* ms);
+int unap_hip_batch_config(unap_hip_batch B, int64_t* block_size, int64_t* wg_per_cu);
 
 #ifdef __cplusplus
 }

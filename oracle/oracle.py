@@ -69,6 +69,10 @@ class CpuSolvePath:
              [_vp, _i64p, _f64p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
               _f64p, _f64p, _f64p, _f64p, ctypes.c_int64, _i64p])
 
+        _sig(L, P + "timed_solve", ctypes.c_int,
+             [_vp, ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, _f64p, _f64p,
+              _f64p, _f64p])
+
     def fn(self, name):
         return getattr(self.lib, self.prefix + name)
 
@@ -186,6 +190,22 @@ class CpuMatrix:
         h = hist[: min(int(hl[0]), hist_cap)].copy()
         return x, {"initial": perf[0], "final": perf[1], "nIterations": int(perf[2]),
                    "converged": bool(perf[3]), "singular": bool(perf[4]), "history": h}
+
+
+def timed_solve(path, M, solver, precond, b, k1, k2, rel_tol, mg=None):
+    """CPU timing of one preconditioner: (perf of the second solve, setup s,
+    cold solve s (k1 iterations max), warm solve s (k2 iterations max))."""
+    x = np.zeros(M.n)
+    b = np.ascontiguousarray(b, np.float64)
+    perf = np.zeros(6)
+    sec = np.zeros(3)
+    mga = None if mg is None else np.asarray(mg, np.int64)
+    rc = path.fn("timed_solve")(M.h, SOLVER[solver], PRECOND[precond], _p(mga, _i64p), k1, k2, rel_tol,
+                                _p(x), _p(b), _p(perf), _p(sec))
+    if rc != 0:
+        raise RuntimeError(f"timed_solve failed: {rc}")
+    return ({"initial": perf[0], "final": perf[1], "nIterations": int(perf[2]), "converged": bool(perf[3])},
+            float(sec[0]), float(sec[1]), float(sec[2]))
 
 
 _oracle = None

@@ -3,6 +3,7 @@
 #define ORACLE_PREFIX uo_
 #include "oracle_capi.h"
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -216,6 +217,39 @@ int uo_mg_solve(void* Av, const int64_t* mg, const double* weights, int64_t maxI
     Perf p = S.solve(x, A, b);
     fillPerf(p, perf);
     copyHistory(h, history, histCap, histLen);
+    return 0;
+}
+
+int uo_timed_solve(void* Av, int solver, int precond, const int64_t* mg, int64_t k1, int64_t k2,
+                   double relTol, double* x, const double* b, double* perf, double* seconds) {
+    const Ldu& A = *(Ldu*)Av;
+    auto t0 = std::chrono::steady_clock::now();
+    std::unique_ptr<Precond> P;
+    if (precond == 4)
+        P.reset(new MGPrecond(A, (int)mg[0], mg[1], mg[2], mg[3], mg[4], mg[5]));
+    else
+        P.reset(makePrecond(A, precond));
+    std::unique_ptr<Solver> S;
+    if (solver == 1)
+        S.reset(new PCG(*P));
+    else if (solver == 2)
+        S.reset(new PBiCGStab(*P));
+    else
+        S.reset(new FCG(*P));
+    S->relTol = relTol;
+    std::vector<double> x1(x, x + A.n), x2(x, x + A.n);
+    auto t1 = std::chrono::steady_clock::now();
+    S->maxIter = k1;
+    S->solve(x1.data(), A, b);
+    auto t2 = std::chrono::steady_clock::now();
+    S->maxIter = k2;
+    Perf p = S->solve(x2.data(), A, b);
+    auto t3 = std::chrono::steady_clock::now();
+    std::memcpy(x, x2.data(), sizeof(double) * A.n);
+    fillPerf(p, perf);
+    seconds[0] = std::chrono::duration<double>(t1 - t0).count();
+    seconds[1] = std::chrono::duration<double>(t2 - t1).count();
+    seconds[2] = std::chrono::duration<double>(t3 - t2).count();
     return 0;
 }
 

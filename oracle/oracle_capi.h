@@ -75,6 +75,15 @@ int OFN(mg_solve)(void* A, const int64_t* mg, const double* weights, int64_t max
                   int64_t minIter, double relTol, double tol, double* x, const double* b,
                   double* perf, double* history, int64_t histCap, int64_t* histLen);
 
+/* Krylov solves timed with a steady clock, for the bench's CPU-baseline leg:
+   seconds[0] = preconditioner setup (including AMG agglomeration),
+   seconds[1] = a first solve capped at k1 iterations (cold: the Chebyshev
+   smoothers' first call estimates eigenvalues), seconds[2] = a second solve
+   from the same x with the same preconditioner capped at k2 iterations.
+   x receives the second solve's result, perf its performance. */
+int OFN(timed_solve)(void* A, int solver, int precond, const int64_t* mg, int64_t k1, int64_t k2,
+                     double relTol, double* x, const double* b, double* perf, double* seconds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,6 +6,7 @@
 #define ORACLE_PREFIX ur_
 #include "oracle_capi.h"
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -292,6 +293,43 @@ int ur_krylov_solve(void* Av, int solver, int precond, const int64_t* mg, int64_
     };
     return solveWithHistory(make, A, maxIter, minIter, relTol, tol, x, b, perf, history, histCap,
                             histLen);
+}
+
+int ur_timed_solve(void* Av, int solver, int precond, const int64_t* mg, int64_t k1, int64_t k2,
+                   double relTol, double* x, const double* b, double* perf, double* seconds) {
+    const lduMatrix& A = *(lduMatrix*)Av;
+    if (precond == 4 && !mg) return -1;
+    if (solver != 1 && solver != 2) return -2;
+    auto t0 = std::chrono::steady_clock::now();
+    std::unique_ptr<matrix::preconditioner> P;
+    if (precond == 4)
+        P.reset(new RefMGPrecond(A, (int)mg[0], mg[1], mg[2], mg[3], mg[4], mg[5]));
+    else
+        P.reset(makePrecond(A, precond));
+    std::unique_ptr<matrix::solver> S;
+    if (solver == 1)
+        S.reset(new PCG(*P));
+    else
+        S.reset(new PBiCGStab(*P));
+    S->SET_relTol(relTol);
+    scalarField bb = wrap(b, A.size());
+    auto t1 = std::chrono::steady_clock::now();
+    // cold: the first preconditioner call runs the smoothers' eigen estimates
+    S->SET_maxIter(k1);
+    scalarField x1 = wrap(x, A.size());
+    S->solve(x1, A, bb);
+    auto t2 = std::chrono::steady_clock::now();
+    // warm: the same preconditioner again from the same start
+    S->SET_maxIter(k2);
+    scalarField x2 = wrap(x, A.size());
+    matrix::solverPerformance p = S->solve(x2, A, bb);
+    auto t3 = std::chrono::steady_clock::now();
+    std::memcpy(x, x2.values(), sizeof(double) * A.size());
+    fillPerf(p, perf);
+    seconds[0] = std::chrono::duration<double>(t1 - t0).count();
+    seconds[1] = std::chrono::duration<double>(t2 - t1).count();
+    seconds[2] = std::chrono::duration<double>(t3 - t2).count();
+    return 0;
 }
 
 int ur_mg_solve(void* Av, const int64_t* mg, const double* weights, int64_t maxIter,

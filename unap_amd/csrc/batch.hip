@@ -1,14 +1,626 @@
-// batch.hip -- many independent LDU systems in one persistent launch.
+// batch.hip -- many independent LDU systems solved in one persistent launch.
+//
+// One workgroup runs one system's whole solve: PCG with a DIC preconditioner
+// (PCG.cpp:35-183, lduDICPrecond.cpp:45-77) or PBiCGStab with DILU
+// (PBiCGStab.cpp:45-459, lduDILUPrecond.cpp:45-86).  The Krylov loop, its dot
+// products and norms run in FP64; the preconditioner sweeps run in FP32 over
+// the system's level schedule with workgroup barriers between levels.
+// Workgroups take systems from an atomic ticket in a largest-first order
+// (longest-processing-time scheduling), and exit when the queue is empty.
+// The block size and workgroups per CU are picked by a small autotuner that
+// times whole batch solves.
 #include "unap_hip.h"
 
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "ops.hpp"
+
+namespace unap_hip {
+
+struct BatchSys {
+    int n, nf, type, nFwd, nBwd;
+    int cellOff;            // cells / rows / internal vectors
+    long long entOff;       // SELL entries
+    int sliceOff;           // slice starts (n_slices + 1 per system)
+    int fwdOff, bwdOff;     // level boundaries
+    int levelMajor;         // forward levels are contiguous internal rows
+};
+
+struct BatchArgs {
+    const BatchSys* sys;
+    const int* order;  // queue order (largest first)
+    int nsys;
+    unsigned* ticket;
+    const long long* sliceStart;
+    const uint32_t* rowInfo;
+    const int* col;
+    const double* val;
+    const double* diag;
+    const float* valP;  // preconditioner coefficients (DIC: upper both sides; DILU: lower / upper)
+    const float* rD;
+    const int* fwdStart;
+    const int* fwdRows;
+    const int* bwdRows;
+    const int* bwdStart;
+    const double* b;
+    double* x;
+    double* v;  // 8 FP64 work vectors, each of total length
+    float* w32;
+    long long total;
+    double relTol, tol;
+    long long maxIter;
+    unap_hip_perf* perf;
+};
+
+__device__ __forceinline__ double bsum(double v, double* red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double a = 0.0;
+    for (int w = 0; w < nw; ++w) a += red[w];
+    __syncthreads();
+    return a;
+}
+__device__ __forceinline__ void bsum2(double& v0, double& v1, double* red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        v0 += __shfl_xor(v0, o, 64);
+        v1 += __shfl_xor(v1, o, 64);
+    }
+    if (lane == 0) {
+        red[wave] = v0;
+        red[16 + wave] = v1;
+    }
+    __syncthreads();
+    double a = 0.0, c = 0.0;
+    for (int w = 0; w < nw; ++w) {
+        a += red[w];
+        c += red[16 + w];
+    }
+    __syncthreads();
+    v0 = a;
+    v1 = c;
+}
+
+struct SysView {
+    DMat<double> A;
+    DMat<float> P;
+    const float* rD;
+    const int* frows;  // null: forward levels are contiguous
+    const int* fs;
+    const int* bs;
+    const int* brows;
+    int nFwd, nBwd, n;
+};
+
+// w32 = M^-1 r in FP32 with the level schedule (DIC or DILU arithmetic of
+// fwdRow / bwdRow, here reading r in FP64).
+__device__ __forceinline__ void batchPrecond(const SysView& s, const double* r, float* w) {
+    for (int k = 0; k < s.nFwd; ++k) {
+        for (int qq = s.fs[k] + threadIdx.x; qq < s.fs[k + 1]; qq += blockDim.x) {
+            const int q = s.frows ? s.frows[qq] : qq;
+            RowEnt<float> e(s.P, q);
+            const float rd = s.rD[q];
+            float acc = rd * (float)r[q];
+            const int mid = e.mid();
+            for (int j = 0; j < mid; ++j) {
+                const long long p = e.at(j);
+                acc = acc - (rd * s.P.val[p]) * w[s.P.col[p]];
+            }
+            w[q] = acc;
+        }
+        __syncthreads();
+    }
+    for (int k = 0; k < s.nBwd; ++k) {
+        for (int q = s.bs[k] + threadIdx.x; q < s.bs[k + 1]; q += blockDim.x) {
+            const int row = s.brows[q];
+            RowEnt<float> e(s.P, row);
+            const float rd = s.rD[row];
+            float acc = w[row];
+            const int mid = e.mid();
+            for (int j = e.len() - 1; j >= mid; --j) {
+                const long long p = e.at(j);
+                acc = acc - (rd * s.P.val[p]) * w[s.P.col[p]];
+            }
+            w[row] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ bool bconv(double fin, double init, double tol, double relTol) {
+    return fin < tol || (relTol > kSmallD && fin <= relTol * init);
+}
+
+__device__ void batchPCG(const BatchArgs& a, const SysView& s, int off, unap_hip_perf* pf, double* red) {
+    const int n = s.n, t = threadIdx.x, nt = blockDim.x;
+    double* x = a.x + off;
+    const double* b = a.b + off;
+    double* r = a.v + off;
+    double* p = a.v + a.total + off;
+    double* w = a.v + 2 * a.total + off;
+    float* w32 = a.w32 + off;
+    double acc = 0.0;
+    for (int i = t; i < n; i += nt) {
+        const double ri = b[i] - rowAmul<double, double>(s.A, i, x);
+        r[i] = ri;
+        p[i] = 0.0;
+        acc += ri * ri;
+    }
+    const double init = sqrt(bsum(acc, red)) + kSmallD;
+    double fin = init, rho = 1.0e37;
+    long long it = 0;
+    int conv = 0, sing = 0;
+    for (;;) {
+        const double rhoOld = rho;
+        batchPrecond(s, r, w32);
+        acc = 0.0;
+        for (int i = t; i < n; i += nt) acc += (double)w32[i] * r[i];
+        rho = bsum(acc, red);
+        if (!(fabs(rho) > kSmallD)) {
+            sing = 1;
+            break;
+        }
+        const double beta = rho / rhoOld;
+        for (int i = t; i < n; i += nt) p[i] = (it == 0) ? (double)w32[i] : (double)w32[i] + beta * p[i];
+        __syncthreads();
+        acc = 0.0;
+        for (int i = t; i < n; i += nt) {
+            const double v = rowAmul<double, double>(s.A, i, p);
+            w[i] = v;
+            acc += v * p[i];
+        }
+        const double alpha = rho / bsum(acc, red);
+        acc = 0.0;
+        for (int i = t; i < n; i += nt) {
+            x[i] += alpha * p[i];
+            const double ri = r[i] - alpha * w[i];
+            r[i] = ri;
+            acc += ri * ri;
+        }
+        fin = sqrt(bsum(acc, red)) + kSmallD;
+        ++it;
+        if (!(it < a.maxIter)) break;
+        if (bconv(fin, init, a.tol, a.relTol)) {
+            conv = 1;
+            break;
+        }
+    }
+    if (t == 0) {
+        pf->initial_residual = init;
+        pf->final_residual = fin;
+        pf->previous_residual = init;
+        pf->n_iterations = it;
+        pf->converged = conv;
+        pf->singular = sing;
+    }
+}
+
+__device__ void batchBiCG(const BatchArgs& a, const SysView& s, int off, unap_hip_perf* pf, double* red) {
+    const int n = s.n, t = threadIdx.x, nt = blockDim.x;
+    const long long T = a.total;
+    double* x = a.x + off;
+    const double* b = a.b + off;
+    double* r = a.v + off;
+    double* r0 = a.v + T + off;
+    double* p = a.v + 2 * T + off;
+    double* Ay = a.v + 3 * T + off;
+    double* sv = a.v + 4 * T + off;
+    double* t_ = a.v + 5 * T + off;
+    double* y = a.v + 6 * T + off;
+    double* z = a.v + 7 * T + off;
+    float* w32 = a.w32 + off;
+    double acc = 0.0;
+    for (int i = t; i < n; i += nt) {
+        const double ri = b[i] - rowAmul<double, double>(s.A, i, x);
+        r[i] = ri;
+        r0[i] = ri;
+        p[i] = 0.0;
+        Ay[i] = 0.0;
+        acc += ri * ri;
+    }
+    const double init = sqrt(bsum(acc, red)) + kSmallD;
+    double fin = init, rho = 0, alpha = 0, omega = 0;
+    long long it = 0;
+    int conv = 0, sing = 0;
+    for (;;) {
+        const double rhoOld = rho;
+        acc = 0.0;
+        for (int i = t; i < n; i += nt) acc += r0[i] * r[i];
+        rho = bsum(acc, red);
+        if (!(fabs(rho) > kSmallD)) {
+            sing = 1;
+            break;
+        }
+        const double beta = (rho / rhoOld) * (alpha / omega);
+        for (int i = t; i < n; i += nt) p[i] = (it == 0) ? r[i] : r[i] + beta * (p[i] - omega * Ay[i]);
+        __syncthreads();
+        batchPrecond(s, p, w32);
+        for (int i = t; i < n; i += nt) y[i] = (double)w32[i];
+        __syncthreads();
+        acc = 0.0;
+        for (int i = t; i < n; i += nt) {
+            const double v = rowAmul<double, double>(s.A, i, y);
+            Ay[i] = v;
+            acc += r0[i] * v;
+        }
+        alpha = rho / bsum(acc, red);
+        acc = 0.0;
+        for (int i = t; i < n; i += nt) {
+            const double si = r[i] - alpha * Ay[i];
+            sv[i] = si;
+            acc += si * si;
+        }
+        fin = sqrt(bsum(acc, red)) + kSmallD;
+        if (bconv(fin, init, a.tol, a.relTol)) {
+            for (int i = t; i < n; i += nt) x[i] += alpha * y[i];
+            ++it;
+            conv = 1;
+            break;
+        }
+        batchPrecond(s, sv, w32);
+        for (int i = t; i < n; i += nt) z[i] = (double)w32[i];
+        __syncthreads();
+        double tt = 0.0, ts = 0.0;
+        for (int i = t; i < n; i += nt) {
+            const double v = rowAmul<double, double>(s.A, i, z);
+            t_[i] = v;
+            tt += v * v;
+            ts += v * sv[i];
+        }
+        bsum2(tt, ts, red);
+        omega = ts / tt;
+        acc = 0.0;
+        for (int i = t; i < n; i += nt) {
+            x[i] += alpha * y[i] + omega * z[i];
+            const double ri = sv[i] - omega * t_[i];
+            r[i] = ri;
+            acc += ri * ri;
+        }
+        fin = sqrt(bsum(acc, red)) + kSmallD;
+        ++it;
+        if (!(it < a.maxIter)) break;
+        if (bconv(fin, init, a.tol, a.relTol)) {
+            conv = 1;
+            break;
+        }
+    }
+    if (t == 0) {
+        pf->initial_residual = init;
+        pf->final_residual = fin;
+        pf->previous_residual = init;
+        pf->n_iterations = it;
+        pf->converged = conv;
+        pf->singular = sing;
+    }
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) kBatchSolve(BatchArgs a) {
+    __shared__ double red[32];
+    __shared__ int ticket;
+    for (;;) {
+        if (threadIdx.x == 0) ticket = (int)atomicAdd(a.ticket, 1u);
+        __syncthreads();
+        const int q = ticket;
+        __syncthreads();
+        if (q >= a.nsys) return;  // the queue is empty: every workgroup ends here
+        const BatchSys d = a.sys[a.order[q]];
+        SysView s;
+        s.n = d.n;
+        s.A = {d.n, a.sliceStart + d.sliceOff, a.rowInfo + d.cellOff, a.col + d.entOff, a.val + d.entOff,
+               a.diag + d.cellOff};
+        s.P = {d.n, a.sliceStart + d.sliceOff, a.rowInfo + d.cellOff, a.col + d.entOff, a.valP + d.entOff,
+               nullptr};
+        s.rD = a.rD + d.cellOff;
+        s.fs = a.fwdStart + d.fwdOff;
+        s.frows = d.levelMajor ? nullptr : a.fwdRows + d.cellOff;
+        s.bs = a.bwdStart + d.bwdOff;
+        s.brows = a.bwdRows + d.cellOff;
+        s.nFwd = d.nFwd;
+        s.nBwd = d.nBwd;
+        if (d.type == UNAP_HIP_SOLVER_PCG)
+            batchPCG(a, s, d.cellOff, a.perf + a.order[q], red);
+        else
+            batchBiCG(a, s, d.cellOff, a.perf + a.order[q], red);
+        __syncthreads();
+    }
+}
+
+__global__ void kResetTicket(unsigned* t) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *t = 0u;
+}
+
+}  // namespace unap_hip
+
+using namespace unap_hip;
+
+struct unap_hip_batch_s {
+    int nsys = 0;
+    long long total = 0;
+    std::vector<BatchSys> sysH;
+    std::vector<int> permH;       // global internal -> global external cell
+    DBuf<BatchSys> sys;
+    DBuf<int> order;
+    DBuf<unsigned> ticket;
+    DBuf<long long> sliceStart;
+    DBuf<uint32_t> rowInfo;
+    DBuf<int> col, fwdStart, fwdRows, bwdRows, bwdStart, perm;
+    DBuf<double> val, diag, x, b, v;
+    DBuf<float> valP, rD, w32;
+    DBuf<unap_hip_perf> perf;
+    int blockSize = 0, wgPerCU = 0;  // 0 = not tuned yet
+    double lastMs = 0;
+    int cus = 0;
+};
+
+namespace {
+
+template <class F>
+int bguard(F&& f) {
+    try {
+        f();
+        setLastError("");
+        return UNAP_HIP_OK;
+    } catch (const Error& e) {
+        setLastError(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        setLastError(e.what());
+        return UNAP_HIP_ERR_INTERNAL;
+    }
+}
+
+void launchBatch(unap_hip_batch_s& B, double relTol, double tol, long long maxIter, int bs, int wgPerCU) {
+    Ctx& c = Ctx::get();
+    BatchArgs a;
+    a.sys = B.sys.get();
+    a.order = B.order.get();
+    a.nsys = B.nsys;
+    a.ticket = B.ticket.get();
+    a.sliceStart = B.sliceStart.get();
+    a.rowInfo = B.rowInfo.get();
+    a.col = B.col.get();
+    a.val = B.val.get();
+    a.diag = B.diag.get();
+    a.valP = B.valP.get();
+    a.rD = B.rD.get();
+    a.fwdStart = B.fwdStart.get();
+    a.fwdRows = B.fwdRows.get();
+    a.bwdRows = B.bwdRows.get();
+    a.bwdStart = B.bwdStart.get();
+    a.b = B.b.get();
+    a.x = B.x.get();
+    a.v = B.v.get();
+    a.w32 = B.w32.get();
+    a.total = B.total;
+    a.relTol = relTol;
+    a.tol = tol;
+    a.maxIter = maxIter;
+    a.perf = B.perf.get();
+    const int grid = std::max(1, std::min(B.nsys, B.cus * wgPerCU));
+    kResetTicket<<<1, 1, 0, c.stream>>>(B.ticket.get());
+    UH_LAUNCHED();
+    switch (bs) {
+        case 256: kBatchSolve<256><<<grid, 256, 0, c.stream>>>(a); break;
+        case 512: kBatchSolve<512><<<grid, 512, 0, c.stream>>>(a); break;
+        default: kBatchSolve<1024><<<grid, 1024, 0, c.stream>>>(a); break;
+    }
+    UH_LAUNCHED();
+}
+
+}  // namespace
+
 extern "C" {
-int unap_hip_batch_create(int64_t, const int64_t*, const int64_t*, const int64_t*, const int64_t*,
-                          const double*, const double*, const double*, const int32_t*, unap_hip_batch*) {
-    return UNAP_HIP_ERR_STATE;
+
+int unap_hip_batch_create(int64_t nsys, const int64_t* cellOff, const int64_t* faceOff, const int64_t* la,
+                          const int64_t* ua, const double* lower, const double* diag, const double* upper,
+                          const int32_t* solver, unap_hip_batch* out) {
+    return bguard([&] {
+        if (!(nsys > 0 && cellOff && faceOff && la && ua && diag && upper && solver && out))
+            throw Error(UNAP_HIP_ERR_ARG, "null or empty batch argument");
+        Ctx& c = Ctx::get();
+        auto* B = new unap_hip_batch_s;
+        std::unique_ptr<unap_hip_batch_s> guardB(B);
+        B->nsys = (int)nsys;
+        std::vector<long long> sliceAll;
+        std::vector<uint32_t> infoAll;
+        std::vector<int> colAll, fwdAll, fwdRowsAll, bwdRowsAll, bwdAll, permAll;
+        std::vector<double> valAll, diagAll;
+        std::vector<float> valPAll, rDAll;
+        long long ent = 0;
+        std::vector<double> work(nsys);
+        for (int64_t s = 0; s < nsys; ++s) {
+            const int64_t n = cellOff[s + 1] - cellOff[s], nf = faceOff[s + 1] - faceOff[s];
+            if (n <= 0 || nf < 0) throw Error(UNAP_HIP_ERR_ARG, "empty system in batch");
+            if (solver[s] != UNAP_HIP_SOLVER_PCG && solver[s] != UNAP_HIP_SOLVER_PBICGSTAB)
+                throw Error(UNAP_HIP_ERR_ARG, "batch systems use PCG (DIC) or PBiCGStab (DILU)");
+            HostLdu A;
+            A.n = (int)n;
+            A.l.resize(nf);
+            A.u.resize(nf);
+            for (int64_t f = 0; f < nf; ++f) {
+                const int64_t lf = la[faceOff[s] + f], uf = ua[faceOff[s] + f];
+                if (lf < 0 || lf >= n || uf < 0 || uf >= n) throw Error(UNAP_HIP_ERR_ARG, "face address out of range");
+                A.l[f] = (int)lf;
+                A.u[f] = (int)uf;
+            }
+            A.d.assign(diag + cellOff[s], diag + cellOff[s] + n);
+            A.up.assign(upper + faceOff[s], upper + faceOff[s] + nf);
+            A.symm = (lower == nullptr || lower == upper);
+            if (!A.symm) A.lo.assign(lower + faceOff[s], lower + faceOff[s] + nf);
+            Layout L = buildLayout(A, 0);  // sweeps dominate: level-major rows
+            if (!L.sweepable)
+                throw Error(UNAP_HIP_ERR_FACE_ORDER, "batch system faces must be owner-sorted with owner < neighbour");
+            BatchSys d;
+            d.n = (int)n;
+            d.nf = (int)nf;
+            d.type = solver[s];
+            d.nFwd = (int)L.fwdStart.size() - 1;
+            d.nBwd = (int)L.bwdStart.size() - 1;
+            d.cellOff = (int)cellOff[s];
+            d.entOff = ent;
+            d.sliceOff = (int)sliceAll.size();
+            d.fwdOff = (int)fwdAll.size();
+            d.bwdOff = (int)bwdAll.size();
+            d.levelMajor = L.fwdRows.empty() ? 1 : 0;
+            B->sysH.push_back(d);
+            sliceAll.insert(sliceAll.end(), L.sliceStart.begin(), L.sliceStart.end());
+            infoAll.insert(infoAll.end(), L.rowInfo.begin(), L.rowInfo.end());
+            colAll.insert(colAll.end(), L.col.begin(), L.col.end());
+            fwdAll.insert(fwdAll.end(), L.fwdStart.begin(), L.fwdStart.end());
+            bwdAll.insert(bwdAll.end(), L.bwdStart.begin(), L.bwdStart.end());
+            bwdRowsAll.insert(bwdRowsAll.end(), L.bwdRows.begin(), L.bwdRows.end());
+            if (L.fwdRows.empty()) {
+                for (int i = 0; i < (int)n; ++i) fwdRowsAll.push_back(i);
+            } else {
+                fwdRowsAll.insert(fwdRowsAll.end(), L.fwdRows.begin(), L.fwdRows.end());
+            }
+            for (int i = 0; i < (int)n; ++i) permAll.push_back((int)cellOff[s] + L.perm[i]);
+            std::vector<double> v;
+            fillSellValues<double>(L, A, false, v);
+            valAll.insert(valAll.end(), v.begin(), v.end());
+            const bool dic = (solver[s] == UNAP_HIP_SOLVER_PCG);
+            std::vector<float> vp;
+            fillSellValues<float>(L, A, dic, vp);
+            valPAll.insert(valPAll.end(), vp.begin(), vp.end());
+            std::vector<double> dd;
+            fillDiag<double>(L, A, dd);
+            diagAll.insert(diagAll.end(), dd.begin(), dd.end());
+            const std::vector<double> rde = dic ? dicReciprocalD(A) : diluReciprocalD(A);
+            for (int i = 0; i < (int)n; ++i) rDAll.push_back((float)rde[L.perm[i]]);
+            ent += L.nnzPadded;
+            work[s] = (double)(n + 2 * nf) * std::cbrt((double)n) * (dic ? 1.0 : 2.0);
+        }
+        B->total = cellOff[nsys];
+        std::vector<int> ord(nsys);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return work[a] > work[b]; });
+        B->sys.upload(B->sysH, c.stream);
+        B->order.upload(ord, c.stream);
+        B->ticket.alloc(1);
+        B->sliceStart.upload(sliceAll, c.stream);
+        B->rowInfo.upload(infoAll, c.stream);
+        B->col.upload(colAll, c.stream);
+        B->fwdStart.upload(fwdAll, c.stream);
+        B->bwdStart.upload(bwdAll, c.stream);
+        B->bwdRows.upload(bwdRowsAll, c.stream);
+        B->fwdRows.upload(fwdRowsAll, c.stream);
+        B->perm.upload(permAll, c.stream);
+        B->val.upload(valAll, c.stream);
+        B->valP.upload(valPAll, c.stream);
+        B->diag.upload(diagAll, c.stream);
+        B->rD.upload(rDAll, c.stream);
+        B->x.alloc(B->total);
+        B->b.alloc(B->total);
+        B->v.alloc(8 * B->total);
+        B->w32.alloc(B->total);
+        B->perf.alloc(nsys);
+        c.sync();
+        int dev = 0;
+        UH_CHECK(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        UH_CHECK(hipGetDeviceProperties(&p, dev));
+        B->cus = p.multiProcessorCount;
+        *out = guardB.release();
+    });
 }
-int unap_hip_batch_destroy(unap_hip_batch) { return UNAP_HIP_OK; }
-int unap_hip_batch_solve(unap_hip_batch, double*, const double*, int, int64_t, double, double, unap_hip_perf*) {
-    return UNAP_HIP_ERR_STATE;
+
+int unap_hip_batch_destroy(unap_hip_batch B) {
+    return bguard([&] { delete B; });
 }
-int unap_hip_batch_last_ms(unap_hip_batch, double*) { return UNAP_HIP_ERR_STATE; }
+
+int unap_hip_batch_solve(unap_hip_batch B, double* x, const double* b, int where, int64_t maxIter, double relTol,
+                         double tol, unap_hip_perf* perf) {
+    return bguard([&] {
+        if (!(B && x && b)) throw Error(UNAP_HIP_ERR_ARG, "null argument");
+        if (maxIter < 0) throw Error(UNAP_HIP_ERR_ARG, "negative iteration limit");
+        Ctx& c = Ctx::get();
+        const long long T = B->total;
+        DBuf<double> stage;
+        const double* bs = b;
+        const double* xs = x;
+        if (where == UNAP_HIP_DEVICE) {
+            UH_CHECK(hipDeviceSynchronize());
+        } else {
+            stage.alloc(2 * T);
+            stage.upload(b, T, c.stream);
+            UH_CHECK(hipMemcpyAsync(stage.get() + T, x, sizeof(double) * T, hipMemcpyHostToDevice, c.stream));
+            bs = stage.get();
+            xs = stage.get() + T;
+        }
+        launchRows(BGather<double, double>{B->perm.get(), bs, B->b.get()}, (int)T);
+        launchRows(BGather<double, double>{B->perm.get(), xs, B->x.get()}, (int)T);
+        // x0 is kept to restart from it while tuning
+        DBuf<double> x0(T);
+        UH_CHECK(hipMemcpyAsync(x0.get(), B->x.get(), sizeof(double) * T, hipMemcpyDeviceToDevice, c.stream));
+        hipEvent_t e0, e1;
+        UH_CHECK(hipEventCreate(&e0));
+        UH_CHECK(hipEventCreate(&e1));
+        auto timed = [&](int bsz, int wg) {
+            UH_CHECK(hipMemcpyAsync(B->x.get(), x0.get(), sizeof(double) * T, hipMemcpyDeviceToDevice, c.stream));
+            UH_CHECK(hipEventRecord(e0, c.stream));
+            launchBatch(*B, relTol, tol, maxIter, bsz, wg);
+            UH_CHECK(hipEventRecord(e1, c.stream));
+            UH_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            UH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            return (double)ms;
+        };
+        if (B->blockSize == 0) {
+            // autotune: block shape x workgroups per CU, one full batch solve each
+            double best = 1e300;
+            for (int bsz : {256, 512, 1024})
+                for (int wg : {1, 2, 4}) {
+                    if (bsz * wg > 2048) continue;
+                    const double ms = timed(bsz, wg);
+                    if (ms < best) {
+                        best = ms;
+                        B->blockSize = bsz;
+                        B->wgPerCU = wg;
+                    }
+                }
+        }
+        B->lastMs = timed(B->blockSize, B->wgPerCU);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (where == UNAP_HIP_DEVICE) {
+            launchRows(BScatter<double, double>{B->perm.get(), B->x.get(), x}, (int)T);
+        } else {
+            launchRows(BScatter<double, double>{B->perm.get(), B->x.get(), stage.get()}, (int)T);
+            UH_CHECK(hipMemcpyAsync(x, stage.get(), sizeof(double) * T, hipMemcpyDeviceToHost, c.stream));
+        }
+        if (perf)
+            UH_CHECK(hipMemcpyAsync(perf, B->perf.get(), sizeof(unap_hip_perf) * B->nsys, hipMemcpyDeviceToHost,
+                                    c.stream));
+        c.sync();
+    });
 }
+
+int unap_hip_batch_last_ms(unap_hip_batch B, double* ms) {
+    return bguard([&] {
+        if (!(B && ms)) throw Error(UNAP_HIP_ERR_ARG, "null argument");
+        *ms = B->lastMs;
+    });
+}
+
+int unap_hip_batch_config(unap_hip_batch B, int64_t* block_size, int64_t* wg_per_cu) {
+    return bguard([&] {
+        if (!B) throw Error(UNAP_HIP_ERR_ARG, "null argument");
+        if (block_size) *block_size = B->blockSize;
+        if (wg_per_cu) *wg_per_cu = B->wgPerCU;
+    });
+}
+
+}  // extern "C"

@@ -17,6 +17,7 @@ struct CoarseArgs {
     DMat<double> A;   // Amul coefficients
     DMat<double> AP;  // preconditioner coefficients (DIC: upper both sides; DILU: as A)
     const double* rD;
+    const int* fwdRows;  // null: forward levels are contiguous internal rows
     const int* fwdStart;
     int nFwd;
     const int* bwdRows;
@@ -94,7 +95,7 @@ __device__ __forceinline__ void wgReduce(const CoarseArgs& a, int n, F term, dou
 __device__ __forceinline__ void wgPrecond(const CoarseArgs& a, double* w, const double* r) {
     for (int k = 0; k < a.nFwd; ++k) {
         for (int q = a.fwdStart[k] + threadIdx.x; q < a.fwdStart[k + 1]; q += blockDim.x)
-            fwdRow<double, double>(a.AP, q, w, r, a.rD);
+            fwdRow<double, double>(a.AP, a.fwdRows ? a.fwdRows[q] : q, w, r, a.rD);
         __syncthreads();
     }
     for (int k = 0; k < a.nBwd; ++k) {
@@ -109,7 +110,7 @@ __device__ __forceinline__ bool wgConv(double fin, double init, double tol, doub
 }
 
 // PCG + DIC (PCG.cpp:35-183) on the coarsest level.
-__global__ void __launch_bounds__(1024) kCoarsePCG(CoarseArgs a, Skip sk) {
+static __global__ void __launch_bounds__(1024) kCoarsePCG(CoarseArgs a, Skip sk) {
     if (sk.on()) return;
     extern __shared__ double lds[];
     const int n = a.A.n;
@@ -159,7 +160,7 @@ __global__ void __launch_bounds__(1024) kCoarsePCG(CoarseArgs a, Skip sk) {
 }
 
 // PBiCGStab + DILU (PBiCGStab.cpp:45-459) on the coarsest level.
-__global__ void __launch_bounds__(1024) kCoarseBiCG(CoarseArgs a, Skip sk) {
+static __global__ void __launch_bounds__(1024) kCoarseBiCG(CoarseArgs a, Skip sk) {
     if (sk.on()) return;
     extern __shared__ double lds[];
     const int n = a.A.n;

@@ -29,6 +29,9 @@ struct Error : std::runtime_error {
 
 #define UH_LAUNCHED() UH_CHECK(hipGetLastError())
 
+// Message returned by unap_hip_last_error() on this thread.
+void setLastError(const std::string& m);
+
 template <typename T>
 struct DBuf {
     T* p = nullptr;
@@ -84,18 +87,15 @@ struct Ctx {
 };
 
 struct Launch {
-    int grid, chunk;
+    int grid;
 };
+// One block per 256 rows up to kMaxBlocks (2048 = 8 per CU); larger row
+// counts loop grid-stride inside the kernel.
 inline Launch rowLaunch(int n) {
-    if (n <= 0) return {1, kBlock};
     long long g = (n + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
     if (g > kMaxBlocks) g = kMaxBlocks;
-    long long chunk = (n + g - 1) / g;
-    chunk = (chunk + 63) / 64 * 64;
-    g = (n + chunk - 1) / chunk;
-    if (g >= 8) g = (g + 7) / 8 * 8;
-    if (g > kMaxBlocks) g = kMaxBlocks;
-    return {(int)g, (int)chunk};
+    return {(int)g};
 }
 
 template <class Body>
@@ -103,7 +103,7 @@ void launchRows(const Body& b, int n, Skip sk = {nullptr, nullptr}) {
     if (n <= 0) return;
     Ctx& c = Ctx::get();
     const Launch L = rowLaunch(n);
-    kRows<Body><<<L.grid, kBlock, 0, c.stream>>>(b, n, L.chunk, sk);
+    kRows<Body><<<L.grid, kBlock, 0, c.stream>>>(b, n, sk);
     UH_LAUNCHED();
 }
 
@@ -113,7 +113,7 @@ template <int NS, class Body, class Fin>
 void launchReduce(const Body& b, int n, const int* order, const Fin& fin, Skip sk = {nullptr, nullptr}) {
     Ctx& c = Ctx::get();
     const Launch L = rowLaunch(n);
-    kRowsReduce<Body, NS><<<L.grid, kBlock, 0, c.stream>>>(b, n, L.chunk, c.partials.get(), sk);
+    kRowsReduce<Body, NS><<<L.grid, kBlock, 0, c.stream>>>(b, n, c.partials.get(), sk);
     UH_LAUNCHED();
     int nb = L.grid;
     if (c.serialReductions) {
@@ -135,7 +135,7 @@ struct DevMatrix {
     DBuf<T> val, diag;
     DBuf<T> valU;  // upper on both sides (DIC on asymmetric storage), lazily
     DBuf<int> perm, iperm;
-    DBuf<int> fwdStart, bwdRows, bwdStart;
+    DBuf<int> fwdRows, fwdStart, bwdRows, bwdStart;  // fwdRows empty: level-major order
     std::vector<int> fwdStartH, bwdStartH;
     bool sweepable = false;
 
@@ -161,6 +161,7 @@ struct DevMatrix {
         fwdStartH = L.fwdStart;
         bwdStartH = L.bwdStart;
         fwdStart.upload(L.fwdStart, s);
+        if (!L.fwdRows.empty()) fwdRows.upload(L.fwdRows, s);
         bwdRows.upload(L.bwdRows, s);
         bwdStart.upload(L.bwdStart, s);
         UH_CHECK(hipStreamSynchronize(s));  // host staging vectors die here
@@ -173,14 +174,16 @@ struct DevMatrix {
         UH_CHECK(hipStreamSynchronize(s));
     }
     void updateValues(const HostLdu& A, const Layout& L, hipStream_t s) {
+        const bool hadU = valU.p != nullptr;
         std::vector<T> v;
         fillSellValues<T>(L, A, false, v);
         val.upload(v, s);
         std::vector<T> d;
         fillDiag<T>(L, A, d);
         diag.upload(d, s);
-        valU.release();
         UH_CHECK(hipStreamSynchronize(s));
+        valU.release();
+        if (hadU) buildValU(A, L, s);
     }
 };
 

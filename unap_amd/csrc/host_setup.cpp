@@ -1,8 +1,12 @@
 // host_setup.cpp -- see host_setup.hpp.
 #include "host_setup.hpp"
 
+#include <cstdlib>
+#include <cstring>
+
 #include <algorithm>
 #include <cmath>
+#include <numeric>
 #include <stdexcept>
 
 namespace unap_hip {
@@ -18,101 +22,95 @@ bool HostLdu::ownerSorted() const {
     return true;
 }
 
-Layout buildLayout(const HostLdu& A) {
-    Layout L;
+namespace {
+
+// Per-row face lists in ascending face id (a single pass over faces keeps
+// that order): side 0 = the row is the neighbour (coefficient lower[f],
+// column owner), side 1 = the row is the owner (upper[f], column neighbour).
+struct RowFaces {
+    std::vector<int> rs, ent, nLowerSide;
+};
+RowFaces rowFaces(const HostLdu& A) {
     const int n = A.n, nf = A.nFaces();
-    L.n = n;
-    L.nf = nf;
-    L.sweepable = A.ownerSorted();
-
-    // Per-row face lists in ascending face id (a single pass over faces keeps
-    // that order): side 0 = the row is the neighbour (coefficient lower[f],
-    // column owner), side 1 = the row is the owner (upper[f], column neighbour).
-    std::vector<int> rs(n + 1, 0);
+    RowFaces R;
+    R.rs.assign(n + 1, 0);
     for (int f = 0; f < nf; ++f) {
-        rs[A.u[f] + 1]++;
-        rs[A.l[f] + 1]++;
+        R.rs[A.u[f] + 1]++;
+        R.rs[A.l[f] + 1]++;
     }
-    for (int c = 0; c < n; ++c) rs[c + 1] += rs[c];
-    std::vector<int> ent(2 * (size_t)nf), fillp(rs.begin(), rs.end() - 1);
-    std::vector<int> nLowerSide(n, 0);
+    for (int c = 0; c < n; ++c) R.rs[c + 1] += R.rs[c];
+    R.ent.resize(2 * (size_t)nf);
+    R.nLowerSide.assign(n, 0);
+    std::vector<int> fillp(R.rs.begin(), R.rs.end() - 1);
     for (int f = 0; f < nf; ++f) {
-        ent[fillp[A.u[f]]++] = 2 * f;  // neighbour side
-        ent[fillp[A.l[f]]++] = 2 * f + 1;
-        nLowerSide[A.u[f]]++;
+        R.ent[fillp[A.u[f]]++] = 2 * f;
+        R.ent[fillp[A.l[f]]++] = 2 * f + 1;
+        R.nLowerSide[A.u[f]]++;
     }
+    return R;
+}
 
-    // Forward level: 1 + max over lower-side neighbours (owners < row);
-    // backward level: 1 + max over upper-side neighbours.  Only meaningful
-    // for owner-sorted faces; otherwise every row is level 0.
-    std::vector<int> lev(n, 0), blev(n, 0);
-    int nlev = 1, nblev = 1;
-    if (L.sweepable) {
-        for (int c = 0; c < n; ++c) {
-            int m = 0;
-            for (int k = rs[c]; k < rs[c + 1]; ++k)
-                if ((ent[k] & 1) == 0) m = std::max(m, lev[A.l[ent[k] >> 1]] + 1);
-            lev[c] = m;
-            nlev = std::max(nlev, m + 1);
-        }
-        for (int c = n - 1; c >= 0; --c) {
-            int m = 0;
-            for (int k = rs[c]; k < rs[c + 1]; ++k)
-                if (ent[k] & 1) m = std::max(m, blev[A.u[ent[k] >> 1]] + 1);
-            blev[c] = m;
-            nblev = std::max(nblev, m + 1);
-        }
-    }
-
-    // Breadth-first order over the cell graph recovers spatial locality that
-    // the caller's numbering may have scrambled.
+// Breadth-first order over the cell graph: recovers the spatial locality
+// that the caller's numbering may have scrambled.
+std::vector<int> bfsOrder(const HostLdu& A, const RowFaces& R) {
+    const int n = A.n;
     std::vector<int> bfs;
     bfs.reserve(n);
-    {
-        std::vector<char> seen(n, 0);
-        for (int s = 0; s < n; ++s) {
-            if (seen[s]) continue;
-            seen[s] = 1;
-            size_t head = bfs.size();
-            bfs.push_back(s);
-            while (head < bfs.size()) {
-                const int c = bfs[head++];
-                for (int k = rs[c]; k < rs[c + 1]; ++k) {
-                    const int f = ent[k] >> 1;
-                    const int nb = (ent[k] & 1) ? A.u[f] : A.l[f];
-                    if (!seen[nb]) {
-                        seen[nb] = 1;
-                        bfs.push_back(nb);
-                    }
+    std::vector<char> seen(n, 0);
+    for (int s = 0; s < n; ++s) {
+        if (seen[s]) continue;
+        seen[s] = 1;
+        size_t head = bfs.size();
+        bfs.push_back(s);
+        while (head < bfs.size()) {
+            const int c = bfs[head++];
+            for (int k = R.rs[c]; k < R.rs[c + 1]; ++k) {
+                const int f = R.ent[k] >> 1;
+                const int nb = (R.ent[k] & 1) ? A.u[f] : A.l[f];
+                if (!seen[nb]) {
+                    seen[nb] = 1;
+                    bfs.push_back(nb);
                 }
             }
         }
     }
+    return bfs;
+}
 
-    // Internal order: forward level major, breadth-first minor.
-    L.fwdStart.assign(nlev + 1, 0);
-    for (int c = 0; c < n; ++c) L.fwdStart[lev[c] + 1]++;
-    for (int k = 0; k < nlev; ++k) L.fwdStart[k + 1] += L.fwdStart[k];
-    {
-        std::vector<int> pos(L.fwdStart.begin(), L.fwdStart.end() - 1);
-        L.perm.assign(n, 0);
-        for (int i = 0; i < n; ++i) {
-            const int c = bfs[i];
-            L.perm[pos[lev[c]]++] = c;
-        }
-    }
+// Rows of one sweep direction grouped by level; `start` gets the boundaries.
+std::vector<int> groupByLevel(const std::vector<int>& levOfInternal, int nlev, std::vector<int>& start) {
+    const int n = (int)levOfInternal.size();
+    start.assign(nlev + 1, 0);
+    for (int i = 0; i < n; ++i) start[levOfInternal[i] + 1]++;
+    for (int k = 0; k < nlev; ++k) start[k + 1] += start[k];
+    std::vector<int> pos(start.begin(), start.end() - 1), rows(n);
+    for (int i = 0; i < n; ++i) rows[pos[levOfInternal[i]]++] = i;
+    return rows;
+}
+
+Layout finishLayout(const HostLdu& A, const RowFaces& R, std::vector<int> perm, const std::vector<int>& lev,
+                    int nlev, const std::vector<int>& blev, int nblev, bool sweepable) {
+    Layout L;
+    const int n = A.n;
+    L.n = n;
+    L.nf = A.nFaces();
+    L.sweepable = sweepable;
+    L.perm = std::move(perm);
     L.iperm.assign(n, 0);
     for (int i = 0; i < n; ++i) L.iperm[L.perm[i]] = i;
 
-    // Backward schedule: internal rows grouped by backward level, ascending.
-    L.bwdStart.assign(nblev + 1, 0);
-    for (int c = 0; c < n; ++c) L.bwdStart[blev[c] + 1]++;
-    for (int k = 0; k < nblev; ++k) L.bwdStart[k + 1] += L.bwdStart[k];
-    {
-        std::vector<int> pos(L.bwdStart.begin(), L.bwdStart.end() - 1);
-        L.bwdRows.assign(n, 0);
-        for (int i = 0; i < n; ++i) L.bwdRows[pos[blev[L.perm[i]]]++] = i;
+    // Sweep schedules over internal rows.  When the internal order is level
+    // major the forward rows are contiguous and fwdRows stays empty.
+    std::vector<int> li(n), bi(n);
+    for (int i = 0; i < n; ++i) {
+        li[i] = lev[L.perm[i]];
+        bi[i] = blev[L.perm[i]];
     }
+    std::vector<int> fr = groupByLevel(li, nlev, L.fwdStart);
+    bool contiguous = true;
+    for (int i = 0; i < n && contiguous; ++i) contiguous = (fr[i] == i);
+    if (!contiguous) L.fwdRows = std::move(fr);
+    L.bwdRows = groupByLevel(bi, nblev, L.bwdStart);
 
     // SELL-64: slice s holds internal rows 64s..64s+63, entry k of a row at
     // sliceStart[s] + 64k + (row & 63).
@@ -123,10 +121,10 @@ Layout buildLayout(const HostLdu& A) {
         int mx = 0;
         for (int r = 64 * s; r < std::min(n, 64 * s + 64); ++r) {
             const int c = L.perm[r];
-            const int len = rs[c + 1] - rs[c];
+            const int len = R.rs[c + 1] - R.rs[c];
             if (len > 0xffff) throw std::runtime_error("row longer than 65535 entries");
             mx = std::max(mx, len);
-            L.rowInfo[r] = (uint32_t)len | ((uint32_t)nLowerSide[c] << 16);
+            L.rowInfo[r] = (uint32_t)len | ((uint32_t)R.nLowerSide[c] << 16);
         }
         L.maxRowLen = std::max(L.maxRowLen, mx);
         L.sliceStart[s + 1] = L.sliceStart[s] + 64LL * mx;
@@ -139,11 +137,11 @@ Layout buildLayout(const HostLdu& A) {
         for (int lane = 0; lane < 64; ++lane) {
             const int r = 64 * s + lane;
             const int c = r < n ? L.perm[r] : -1;
-            const int len = r < n ? rs[c + 1] - rs[c] : 0;
+            const int len = r < n ? R.rs[c + 1] - R.rs[c] : 0;
             for (int k = 0; k < mx; ++k) {
                 const long long p = L.sliceStart[s] + 64LL * k + lane;
                 if (k < len) {
-                    const int e = ent[rs[c] + k];
+                    const int e = R.ent[R.rs[c] + k];
                     const int f = e >> 1;
                     L.col[p] = L.iperm[(e & 1) ? A.u[f] : A.l[f]];
                     L.src[p] = e;
@@ -155,6 +153,88 @@ Layout buildLayout(const HostLdu& A) {
         }
     }
     return L;
+}
+
+// Forward level: 1 + max over lower-side neighbours (owners < row); backward
+// level: 1 + max over upper-side neighbours.  Only meaningful for owner-sorted
+// faces; otherwise every row is level 0.
+void levels(const HostLdu& A, const RowFaces& R, bool sweepable, std::vector<int>& lev, int& nlev,
+            std::vector<int>& blev, int& nblev) {
+    const int n = A.n;
+    lev.assign(n, 0);
+    blev.assign(n, 0);
+    nlev = nblev = 1;
+    if (!sweepable) return;
+    for (int c = 0; c < n; ++c) {
+        int m = 0;
+        for (int k = R.rs[c]; k < R.rs[c + 1]; ++k)
+            if ((R.ent[k] & 1) == 0) m = std::max(m, lev[A.l[R.ent[k] >> 1]] + 1);
+        lev[c] = m;
+        nlev = std::max(nlev, m + 1);
+    }
+    for (int c = n - 1; c >= 0; --c) {
+        int m = 0;
+        for (int k = R.rs[c]; k < R.rs[c + 1]; ++k)
+            if (R.ent[k] & 1) m = std::max(m, blev[A.u[R.ent[k] >> 1]] + 1);
+        blev[c] = m;
+        nblev = std::max(nblev, m + 1);
+    }
+}
+
+}  // namespace
+
+int forcedOrderMode() {
+    static const int mode = [] {
+        const char* e = std::getenv("UNAP_HIP_CELL_ORDER");
+        if (!e) return -1;
+        if (std::strcmp(e, "level") == 0) return 0;
+        if (std::strcmp(e, "bfs") == 0) return 1;
+        if (std::strcmp(e, "natural") == 0) return 2;
+        return -1;
+    }();
+    return mode;
+}
+
+void levelCounts(const HostLdu& A, int& nFwd, int& nBwd) {
+    nFwd = nBwd = 0;
+    if (!A.ownerSorted()) return;
+    const RowFaces R = rowFaces(A);
+    std::vector<int> lev, blev;
+    levels(A, R, true, lev, nFwd, blev, nBwd);
+}
+
+Layout buildLayout(const HostLdu& A, int orderMode) {
+    const int forced = forcedOrderMode();
+    if (forced >= 0) orderMode = forced;
+    const RowFaces R = rowFaces(A);
+    const bool sweepable = A.ownerSorted();
+    std::vector<int> lev, blev;
+    int nlev, nblev;
+    levels(A, R, sweepable, lev, nlev, blev, nblev);
+    std::vector<int> perm(A.n);
+    if (orderMode == 2) {
+        std::iota(perm.begin(), perm.end(), 0);  // the caller's order
+    } else {
+        const std::vector<int> bfs = bfsOrder(A, R);
+        if (orderMode == 1) {
+            perm = bfs;
+        } else {  // forward level major, breadth-first minor
+            std::vector<int> start(nlev + 1, 0);
+            for (int c = 0; c < A.n; ++c) start[lev[c] + 1]++;
+            for (int k = 0; k < nlev; ++k) start[k + 1] += start[k];
+            for (int i = 0; i < A.n; ++i) perm[start[lev[bfs[i]]]++] = bfs[i];
+        }
+    }
+    return finishLayout(A, R, std::move(perm), lev, nlev, blev, nblev, sweepable);
+}
+
+Layout buildLayoutWithOrder(const HostLdu& A, const std::vector<int>& perm) {
+    const RowFaces R = rowFaces(A);
+    const bool sweepable = A.ownerSorted();
+    std::vector<int> lev, blev;
+    int nlev, nblev;
+    levels(A, R, sweepable, lev, nlev, blev, nblev);
+    return finishLayout(A, R, perm, lev, nlev, blev, nblev, sweepable);
 }
 
 template <typename T>

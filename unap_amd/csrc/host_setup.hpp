@@ -40,14 +40,27 @@ struct Layout {
     std::vector<uint32_t> rowInfo;      // len | nLowerSide << 16, internal order
     std::vector<int> col;               // padded, internal ids (padding -> own row)
     std::vector<int> src;               // padded: 2*face + side (0 lower, 1 upper), -1 pad
-    std::vector<int> fwdStart;          // forward levels: internal rows [fwdStart[k], fwdStart[k+1])
+    std::vector<int> fwdRows;           // internal rows by forward level (empty: level-major order)
+    std::vector<int> fwdStart;          // forward levels: fwdRows[fwdStart[k] .. fwdStart[k+1]) or,
+                                        // with fwdRows empty, internal rows [fwdStart[k], fwdStart[k+1])
     std::vector<int> bwdRows;           // internal rows sorted by backward level
     std::vector<int> bwdStart;          // backward levels: bwdRows[bwdStart[k] .. bwdStart[k+1])
     int maxRowLen = 0;
     bool sweepable = false;             // ownerSorted(): sweeps reproduce the reference
 };
 
-Layout buildLayout(const HostLdu& A);
+// orderMode 0: forward-level major, breadth-first within a level (sweeps
+// read contiguous rows); 1: breadth-first only (Amul and Chebyshev stream at
+// twice the level-major rate; sweeps gather their rows through fwdRows);
+// 2: the caller's order.
+Layout buildLayout(const HostLdu& A, int orderMode);
+// Order forced for every layout by the environment variable
+// UNAP_HIP_CELL_ORDER ("level" 0, "bfs" 1, "natural" 2), or -1 when unset.
+int forcedOrderMode();
+// Forward / backward sweep level counts (0 when the faces are not owner-sorted).
+void levelCounts(const HostLdu& A, int& nFwd, int& nBwd);
+// An explicit internal order (perm: internal -> external).
+Layout buildLayoutWithOrder(const HostLdu& A, const std::vector<int>& perm);
 
 // Values in the device layout.  useUpperBothSides: DIC reads upper on both
 // sweep directions (lduDICPrecond.cpp:68-76) even for asymmetric storage.

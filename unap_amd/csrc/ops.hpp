@@ -208,7 +208,7 @@ struct BBicgXR {  // PBiCGStab.cpp:388-396
 };
 // early exit x += alpha yA (PBiCGStab.cpp:298-312), only in the iteration that
 // took it
-__global__ void kBicgEarlyX(double* x, const double* y, const KState* st, long long it, int n) {
+static __global__ void kBicgEarlyX(double* x, const double* y, const KState* st, long long it, int n) {
     if (st->exitIt != it) return;
     const double al = st->alpha;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)

@@ -15,24 +15,54 @@
 namespace unap_hip {
 
 // ------------------------------------------------------------------ matrix
-struct LduObj {
-    HostLdu host;
-    Layout layout;
+// Cell orders of the device copies.  Sweeps (GS, DIC, DILU) run level by
+// level and want each level's rows contiguous (level-major order); Amul and
+// every row-parallel kernel want neighbours close together (breadth-first
+// order).  On a 256^3 Poisson matrix: Amul 4.1 TB/s breadth-first against
+// 1.9 TB/s level-major; DIC 0.82 TB/s level-major against 0.45 TB/s
+// breadth-first.  Each consumer asks for the order its hot kernel wants.
+enum CellOrder { ORDER_SWEEP = 0, ORDER_STREAM = 1 };
+inline int layoutMode(int order) {
+    const int forced = forcedOrderMode();
+    return forced >= 0 ? forced : (order == ORDER_SWEEP ? 0 : 1);
+}
+
+struct AgglHost;
+struct LduView {
+    std::unique_ptr<Layout> layout;
     std::unique_ptr<DevMatrix<double>> d64;
     std::unique_ptr<DevMatrix<float>> d32;
-    DevMatrix<double>& dev64() {
-        if (!d64) {
-            d64.reset(new DevMatrix<double>);
-            d64->build(host, layout, Ctx::get().stream);
-        }
-        return *d64;
+};
+struct LduObj {
+    HostLdu host;
+    int nFwdLevels = 0, nBwdLevels = 0;
+    // agglomeration built from the default weights, shared by every
+    // preconditioner / solver of this matrix with the same level limits
+    std::shared_ptr<AgglHost> agglCache;
+    int agglMaxLevels = -1, agglNCoarsest = -1, agglNormalised = -1;
+    LduView views[3];  // by layout mode, built on first use
+    const Layout& layout(int order) {
+        LduView& v = views[layoutMode(order)];
+        if (!v.layout) v.layout.reset(new Layout(buildLayout(host, layoutMode(order))));
+        return *v.layout;
     }
-    DevMatrix<float>& dev32() {
-        if (!d32) {
-            d32.reset(new DevMatrix<float>);
-            d32->build(host, layout, Ctx::get().stream);
+    DevMatrix<double>& dev64(int order) {
+        LduView& v = views[layoutMode(order)];
+        if (!v.d64) {
+            const Layout& L = layout(order);
+            v.d64.reset(new DevMatrix<double>);
+            v.d64->build(host, L, Ctx::get().stream);
         }
-        return *d32;
+        return *v.d64;
+    }
+    DevMatrix<float>& dev32(int order) {
+        LduView& v = views[layoutMode(order)];
+        if (!v.d32) {
+            const Layout& L = layout(order);
+            v.d32.reset(new DevMatrix<float>);
+            v.d32->build(host, L, Ctx::get().stream);
+        }
+        return *v.d32;
     }
     int n() const { return host.n; }
 };
@@ -71,7 +101,7 @@ void runSchedule(SweepArgs<T, TV> a, const std::vector<int>& levStartH, Skip sk)
 template <typename T>
 void gsSmooth(DevMatrix<T>& M, T* x, const T* b, int nSweeps, Skip sk) {
     if (!M.sweepable) throw Error(-3, "Gauss-Seidel needs faces sorted by owner with owner < neighbour");
-    SweepArgs<T, T> a{M.view(), x, b, nullptr, nullptr, M.fwdStart.get()};
+    SweepArgs<T, T> a{M.view(), x, b, nullptr, M.fwdRows.p, M.fwdStart.get()};
     const int nl = (int)M.fwdStartH.size() - 1;
     if (tailStart(M.fwdStartH) == 0) {
         kSweepTail<T, T, SWEEP_GS><<<1, 1024, 0, Ctx::get().stream>>>(a, 0, nl, nSweeps, sk);
@@ -86,7 +116,7 @@ template <typename T, typename TV>
 void luSweeps(DevMatrix<T>& M, bool dic, TV* w, const TV* r, const TV* rD, Skip sk) {
     if (!M.sweepable) throw Error(-3, "DIC/DILU needs faces sorted by owner with owner < neighbour");
     const DMat<T> Av = dic ? M.viewU() : M.view();
-    SweepArgs<T, TV> f{Av, w, r, rD, nullptr, M.fwdStart.get()};
+    SweepArgs<T, TV> f{Av, w, r, rD, M.fwdRows.p, M.fwdStart.get()};
     runSchedule<T, TV, SWEEP_FWD>(f, M.fwdStartH, sk);
     SweepArgs<T, TV> bk{Av, w, nullptr, rD, M.bwdRows.get(), M.bwdStart.get()};
     runSchedule<T, TV, SWEEP_BWD>(bk, M.bwdStartH, sk);
@@ -155,18 +185,26 @@ struct SmootherObj {
 // ----------------------------------------------------- agglomeration levels
 struct AgglHost {
     std::vector<CoarseLevel> levels;
-    std::vector<Layout> layouts;
+    std::vector<Layout> layouts[3];  // per layout mode, built on first use
+    const std::vector<Layout>& layoutsFor(int order) {
+        std::vector<Layout>& v = layouts[layoutMode(order)];
+        if (v.size() != levels.size()) {
+            v.clear();
+            for (auto& L : levels) v.push_back(buildLayout(L.A, layoutMode(order)));
+        }
+        return v;
+    }
 };
 inline void buildAggl(const HostLdu& fine, const std::vector<double>& w, int maxLevels, int nCoarsest,
                       AgglHost& out) {
     out.levels = agglomerate(fine, w, maxLevels, nCoarsest);
-    out.layouts.clear();
-    for (auto& L : out.levels) out.layouts.push_back(buildLayout(L.A));
+    for (auto& v : out.layouts) v.clear();
 }
 
 // -------------------------------------------------------------- hierarchy
 struct AmgParams {
     int smoother = 1, nPre = 0, nPost = 2, nFinest = 2, nVcycles = 2, precision = 64;
+    int growth = 1;  // coarse level l smooths nPost + l (pre: nPre + l) times
     int maxLevels = 50, nCoarsest = 50;
 };
 
@@ -185,6 +223,7 @@ struct Hierarchy {
     DBuf<double> coarseSrc64, coarseCorr64;           // FP32 path: coarsest in FP64
     std::vector<DBuf<T>> pre;                         // pre-smoothed corrections (nPre > 0)
     AmgParams prm;
+    int order = ORDER_STREAM;  // GS hierarchies sweep: ORDER_SWEEP
     bool scale = true;
     int L = 0;
     // smoothers: index 0 finest, l+1 coarse level l
@@ -214,6 +253,7 @@ struct Hierarchy {
 // ------------------------------------------------------------ preconditioner
 struct PrecondObj {
     int type = 0;
+    int order = ORDER_STREAM;  // cell order of the vectors it works on
     LduObj* A = nullptr;
     DBuf<double> rD;                 // internal order
     std::vector<double> rDext;       // external order (rD())

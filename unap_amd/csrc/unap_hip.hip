@@ -16,6 +16,7 @@
 namespace unap_hip {
 
 static thread_local std::string g_lastError;
+void setLastError(const std::string& m) { g_lastError = m; }
 
 Ctx& Ctx::get() {
     static Ctx* c = nullptr;
@@ -48,11 +49,11 @@ template <typename T>
 struct Fine;
 template <>
 struct Fine<double> {
-    static DevMatrix<double>& get(LduObj& A) { return A.dev64(); }
+    static DevMatrix<double>& get(LduObj& A, int order) { return A.dev64(order); }
 };
 template <>
 struct Fine<float> {
-    static DevMatrix<float>& get(LduObj& A) { return A.dev32(); }
+    static DevMatrix<float>& get(LduObj& A, int order) { return A.dev32(order); }
 };
 
 // Dense inverse by Gauss-Jordan elimination with partial pivoting (host, FP64).
@@ -98,9 +99,11 @@ template <typename T>
 void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgParams& p) {
     Ctx& c = Ctx::get();
     fineObj = &A;
-    fine = &Fine<T>::get(A);
+    order = (p.smoother == UNAP_HIP_SMOOTHER_GS) ? ORDER_SWEEP : ORDER_STREAM;
+    fine = &Fine<T>::get(A, order);
     ag = agg;
     prm = p;
+    const std::vector<Layout>& lays = ag->layoutsFor(order);
     L = (int)ag->levels.size();
     if (L == 0)
         throw Error(-4, "agglomeration produced no coarse level (matrix smaller than the coarsest-level size)");
@@ -114,8 +117,8 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     mats.resize(L);
     work.resize(L);
     for (int l = 0; l < L; ++l) {
-        const Layout& fl = (l == 0) ? A.layout : ag->layouts[l - 1];
-        const Layout& cl = ag->layouts[l];
+        const Layout& fl = (l == 0) ? A.layout(order) : lays[l - 1];
+        const Layout& cl = lays[l];
         Transfer tr = buildTransfer(fl, cl, ag->levels[l].restrictMap);
         cs.emplace_back();
         ch.emplace_back();
@@ -143,7 +146,7 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     mgst.alloc(1);
 
     const HostLdu& CA = ag->levels[L - 1].A;
-    const Layout& CL = ag->layouts[L - 1];
+    const Layout& CL = lays[L - 1];
     if (!fp32) {
         // reference path: PCG+DIC / PBiCGStab+DILU to relTol 1e-2 in one workgroup
         coarsest64.reset();
@@ -242,6 +245,7 @@ void Hierarchy<T>::coarsestSolve(Skip sk) {
     a.A = cm->view();
     a.AP = CA.symm ? cm->viewU() : cm->view();
     a.rD = rDc.get();
+    a.fwdRows = cm->fwdRows.p;
     a.fwdStart = cm->fwdStart.get();
     a.nFwd = (int)cm->fwdStartH.size() - 1;
     a.bwdRows = cm->bwdRows.get();
@@ -274,7 +278,7 @@ void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, const double* castFr
         const int n = nl(l);
         if (prm.nPre) {
             launchRows(BFill<T>{W.a.get(), (T)0}, n, sk);
-            smooth(l + 1, W.a.get(), W.b.get(), prm.nPre + l, sk);
+            smooth(l + 1, W.a.get(), W.b.get(), prm.nPre + prm.growth * l, sk);
             if (scale && l < coarsest - 1) {
                 launchReduce<2>(BSfAmul<T>{mat(l).view(), W.a.get(), W.b.get(), W.c.get()}, n,
                                 mat(l).iperm.get(), FinSF{mgst.get(), l}, sk);
@@ -302,7 +306,7 @@ void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, const double* castFr
         } else if (prm.nPre) {
             launchRows(BAddTo<T>{W.a.get(), pre[l].get()}, n, sk);
         }
-        smooth(l + 1, W.a.get(), W.b.get(), prm.nPost + l, sk);
+        smooth(l + 1, W.a.get(), W.b.get(), prm.nPost + prm.growth * l, sk);
     }
     const int n = fine->n;
     launchRows(BProlong<T>{f2c[0].get(), work[0].a.get(), fw.a.get()}, n, sk);
@@ -357,7 +361,7 @@ void applyPrecond64(PrecondObj& P, const double* r, double* w, Skip sk) {
             break;
         case UNAP_HIP_PRECOND_DIC:
         case UNAP_HIP_PRECOND_DILU:
-            luSweeps<double, double>(P.A->dev64(), P.type == UNAP_HIP_PRECOND_DIC, w, r, P.rD.get(), sk);
+            luSweeps<double, double>(P.A->dev64(P.order), P.type == UNAP_HIP_PRECOND_DIC, w, r, P.rD.get(), sk);
             break;
         case UNAP_HIP_PRECOND_AMG:
             if (!P.amg64) throw Error(-4, "FP32 AMG preconditioner needs the FCG solver");
@@ -411,10 +415,12 @@ static int nextBatch(const KState& h, int cur) {
 struct Driver {
     SolverObj& S;
     LduObj& A;
+    DevMatrix<double>& M;  // the matrix in the preconditioner's cell order
     SolveBufs& B;
     KState* st;
     int n;
-    Driver(SolverObj& s, LduObj& a, SolveBufs& b) : S(s), A(a), B(b), st(b.st.get()), n(a.n()) {}
+    Driver(SolverObj& s, LduObj& a, DevMatrix<double>& m, SolveBufs& b)
+        : S(s), A(a), M(m), B(b), st(b.st.get()), n(a.n()) {}
 
     template <class Enqueue>
     void iterate(Enqueue&& oneIteration) {
@@ -430,7 +436,6 @@ struct Driver {
 
     void pcg() {
         double *x = B.x.get(), *b = B.b.get(), *r = B.v[0].get(), *w = B.v[1].get(), *p = B.v[2].get();
-        DevMatrix<double>& M = A.dev64();
         launchReduce<1>(BResidual<double>{M.view(), x, b, r, nullptr}, n, M.iperm.get(), FinInitRes{st, 0});
         const Skip sk{&st->done, nullptr};
         PrecondObj* P = S.P;
@@ -460,7 +465,6 @@ struct Driver {
         double *x = B.x.get(), *b = B.b.get();
         double *r = B.v[0].get(), *r0 = B.v[1].get(), *p = B.v[2].get(), *y = B.v[3].get();
         double *Ay = B.v[4].get(), *s = B.v[5].get(), *z = B.v[6].get(), *t = B.v[7].get();
-        DevMatrix<double>& M = A.dev64();
         launchReduce<1>(BResidual<double>{M.view(), x, b, r, r0}, n, M.iperm.get(), FinInitRes{st, 0});
         launchRows(BFill<double>{Ay, 0.0}, n);
         launchRows(BFill<double>{p, 0.0}, n);
@@ -485,7 +489,6 @@ struct Driver {
     template <typename TZ>
     void fcgWith(TZ* z0, TZ* z1, std::function<void(const double*, TZ*, Skip)> prec) {
         double *x = B.x.get(), *b = B.b.get(), *r = B.v[0].get(), *q = B.v[1].get(), *p = B.v[2].get();
-        DevMatrix<double>& M = A.dev64();
         launchReduce<1>(BResidual<double>{M.view(), x, b, r, nullptr}, n, M.iperm.get(), FinInitRes{st, 0});
         const Skip sk{&st->done, nullptr};
         TZ* z = z0;
@@ -513,7 +516,6 @@ struct Driver {
 
     void mg(Hierarchy<double>& H) {
         double *x = B.x.get(), *b = B.b.get(), *fres = B.v[0].get();
-        DevMatrix<double>& M = A.dev64();
         H.initSmoothers();
         launchReduce<1>(BResidual<double>{M.view(), x, b, fres, nullptr}, n, M.iperm.get(), FinInitRes{st, 1});
         const Skip sk{&st->done, nullptr};
@@ -541,6 +543,21 @@ static std::shared_ptr<AgglHost> makeAggl(const HostLdu& A, const double* weight
     auto ag = std::make_shared<AgglHost>();
     buildAggl(A, w, maxLevels, nCoarsest, *ag);
     return ag;
+}
+
+// Default-weight hierarchies are cached on the matrix (coefficients are part
+// of the key only through set_coeffs, which clears the cache).
+static std::shared_ptr<AgglHost> agglFor(LduObj& L, const double* weights, bool normalise, int maxLevels,
+                                         int nCoarsest) {
+    if (weights) return makeAggl(L.host, weights, normalise, maxLevels, nCoarsest);
+    if (L.agglCache && L.agglMaxLevels == maxLevels && L.agglNCoarsest == nCoarsest &&
+        L.agglNormalised == (int)normalise)
+        return L.agglCache;
+    L.agglCache = makeAggl(L.host, nullptr, normalise, maxLevels, nCoarsest);
+    L.agglMaxLevels = maxLevels;
+    L.agglNCoarsest = nCoarsest;
+    L.agglNormalised = (int)normalise;
+    return L.agglCache;
 }
 
 }  // namespace unap_hip
@@ -666,12 +683,7 @@ int unap_hip_ldu_create(int64_t n, int64_t nf, const int64_t* la, const int64_t*
         A.up.assign(upper, upper + nf);
         A.symm = (lower == nullptr || lower == upper);
         if (!A.symm) A.lo.assign(lower, lower + nf);
-        try {
-            h->o.layout = buildLayout(A);
-        } catch (...) {
-            delete h;
-            throw;
-        }
+        levelCounts(A, h->o.nFwdLevels, h->o.nBwdLevels);
         *out = h;
     });
 }
@@ -686,9 +698,8 @@ int unap_hip_ldu_info(unap_hip_ldu A, int64_t* n, int64_t* nf, int* symm, int64_
         if (n) *n = A->o.host.n;
         if (nf) *nf = A->o.host.nFaces();
         if (symm) *symm = A->o.host.symm ? 1 : 0;
-        const bool sw = A->o.layout.sweepable;
-        if (fl) *fl = sw ? (int64_t)A->o.layout.fwdStart.size() - 1 : 0;
-        if (bl) *bl = sw ? (int64_t)A->o.layout.bwdStart.size() - 1 : 0;
+        if (fl) *fl = A->o.nFwdLevels;
+        if (bl) *bl = A->o.nBwdLevels;
     });
 }
 
@@ -703,8 +714,11 @@ int unap_hip_ldu_set_coeffs(unap_hip_ldu A, const double* diag, const double* up
         H.lo.clear();
         if (!H.symm) H.lo.assign(lower, lower + H.nFaces());
         Ctx& c = Ctx::get();
-        if (A->o.d64) A->o.d64->updateValues(H, A->o.layout, c.stream);
-        if (A->o.d32) A->o.d32->updateValues(H, A->o.layout, c.stream);
+        A->o.agglCache.reset();
+        for (LduView& v : A->o.views) {
+            if (v.d64) v.d64->updateValues(H, *v.layout, c.stream);
+            if (v.d32) v.d32->updateValues(H, *v.layout, c.stream);
+        }
     });
 }
 
@@ -712,7 +726,7 @@ int unap_hip_ldu_spmv(unap_hip_ldu A, const double* psi, double* Apsi, int where
     return guard([&] {
         checkLdu(A);
         need(psi && Apsi, "null vector");
-        DevMatrix<double>& M = A->o.dev64();
+        DevMatrix<double>& M = A->o.dev64(ORDER_STREAM);
         DBuf<double> xi(M.n), yi(M.n);
         toInternal(M, psi, where, xi.get(), stageBuf());
         launchRows(BAmul<double>{M.view(), xi.get(), yi.get()}, M.n);
@@ -731,6 +745,7 @@ int unap_hip_amg_params_default(unap_hip_amg_params* p) {
         p->n_post_sweeps = 2;
         p->n_finest_sweeps = 2;
         p->precision = 64;
+        p->level_sweep_growth = 1;
     });
 }
 
@@ -745,6 +760,8 @@ static AmgParams toParams(const unap_hip_amg_params* p) {
     a.nPost = (int)p->n_post_sweeps;
     a.nFinest = (int)p->n_finest_sweeps;
     a.precision = (int)p->precision;
+    a.growth = (int)p->level_sweep_growth;
+    need(a.growth == 0 || a.growth == 1, "level_sweep_growth must be 0 or 1");
     need(a.smoother == 1 || a.smoother == 2, "smoother must be GS (1) or Chebyshev (2)");
     need(a.precision == 64 || a.precision == 32, "precision must be 64 or 32");
     need(a.nVcycles >= 1 && a.nPre >= 0 && a.nPost >= 0 && a.nFinest >= 0, "bad sweep counts");
@@ -763,25 +780,27 @@ int unap_hip_precond_create(unap_hip_ldu A, int type, const unap_hip_amg_params*
         o.type = type;
         o.A = &A->o;
         const HostLdu& H = A->o.host;
-        const Layout& L = A->o.layout;
         Ctx& c = Ctx::get();
+        if (type == UNAP_HIP_PRECOND_DIC || type == UNAP_HIP_PRECOND_DILU) o.order = ORDER_SWEEP;
         if (type == UNAP_HIP_PRECOND_DIAG || type == UNAP_HIP_PRECOND_DIC || type == UNAP_HIP_PRECOND_DILU) {
+            const Layout& L = A->o.layout(o.order);
             if (type == UNAP_HIP_PRECOND_DIAG) {
                 o.rDext.resize(H.n);
                 for (int i = 0; i < H.n; ++i) o.rDext[i] = 1.0 / H.d[i];
             } else {
-                if (!L.sweepable)
+                if (!H.ownerSorted())
                     throw Error(UNAP_HIP_ERR_FACE_ORDER, "DIC/DILU need faces sorted by owner with owner < neighbour");
                 o.rDext = (type == UNAP_HIP_PRECOND_DIC) ? dicReciprocalD(H) : diluReciprocalD(H);
             }
             std::vector<double> rin(H.n);
             for (int i = 0; i < H.n; ++i) rin[i] = o.rDext[L.perm[i]];
             o.rD.upload(rin, c.stream);
-            if (type == UNAP_HIP_PRECOND_DIC) A->o.dev64().buildValU(H, L, c.stream);
+            if (type == UNAP_HIP_PRECOND_DIC) A->o.dev64(o.order).buildValU(H, L, c.stream);
             c.sync();
         } else if (type == UNAP_HIP_PRECOND_AMG) {
             AmgParams prm = toParams(amg);
-            auto ag = makeAggl(H, fw, false, prm.maxLevels, prm.nCoarsest);
+            o.order = (prm.smoother == UNAP_HIP_SMOOTHER_GS) ? ORDER_SWEEP : ORDER_STREAM;
+            auto ag = agglFor(A->o, fw, false, prm.maxLevels, prm.nCoarsest);
             if (prm.precision == 32) {
                 o.amg32.reset(new Hierarchy<float>);
                 o.amg32->build(A->o, ag, prm);
@@ -802,7 +821,7 @@ int unap_hip_precond_apply(unap_hip_precond P, const double* r, double* w, int w
     return guard([&] {
         need(P && r && w, "null argument");
         PrecondObj& o = P->o;
-        DevMatrix<double>& M = o.A->dev64();
+        DevMatrix<double>& M = o.A->dev64(o.order);
         DBuf<double> ri(M.n), wi(M.n);
         toInternal(M, r, where, ri.get(), stageBuf());
         if (o.amg32) {
@@ -852,7 +871,7 @@ int unap_hip_smoother_smooth(unap_hip_smoother S, unap_hip_ldu A, double* x, con
     return guard([&] {
         need(S && A && x && b, "null argument");
         need(nSweeps >= 0, "negative sweep count");
-        DevMatrix<double>& M = A->o.dev64();
+        DevMatrix<double>& M = A->o.dev64(S->o.type == UNAP_HIP_SMOOTHER_GS ? ORDER_SWEEP : ORDER_STREAM);
         DBuf<double> xi(M.n), bi(M.n);
         toInternal(M, x, where, xi.get(), stageBuf());
         toInternal(M, b, where, bi.get(), stageBuf());
@@ -875,7 +894,7 @@ int unap_hip_smoother_smooth(unap_hip_smoother S, unap_hip_ldu A, double* x, con
 int unap_hip_eigen_estimate(unap_hip_ldu A, double* x, const double* b, int64_t nPCGs, int where, double* lam) {
     return guard([&] {
         need(A && x && b && lam, "null argument");
-        DevMatrix<double>& M = A->o.dev64();
+        DevMatrix<double>& M = A->o.dev64(ORDER_STREAM);
         DBuf<double> xi(M.n), bi(M.n), p(M.n), w(M.n), r(M.n);
         toInternal(M, x, where, xi.get(), stageBuf());
         toInternal(M, b, where, bi.get(), stageBuf());
@@ -969,8 +988,9 @@ static void ensureTransfers(unap_hip_aggl G) {
     if (!G->cs.empty()) return;
     Ctx& c = Ctx::get();
     for (size_t l = 0; l < G->ag->levels.size(); ++l) {
-        const Layout& fl = (l == 0) ? G->A->layout : G->ag->layouts[l - 1];
-        Transfer tr = buildTransfer(fl, G->ag->layouts[l], G->ag->levels[l].restrictMap);
+        const std::vector<Layout>& lays = G->ag->layoutsFor(ORDER_STREAM);
+        const Layout& fl = (l == 0) ? G->A->layout(ORDER_STREAM) : lays[l - 1];
+        Transfer tr = buildTransfer(fl, lays[l], G->ag->levels[l].restrictMap);
         G->cs.emplace_back();
         G->ch.emplace_back();
         G->f2c.emplace_back();
@@ -987,8 +1007,9 @@ int unap_hip_aggl_restrict_field(unap_hip_aggl G, int64_t l, const double* ff, d
         need(ff && cf, "null vector");
         ensureTransfers(G);
         Ctx& c = Ctx::get();
-        const Layout& fl = (l == 0) ? G->A->layout : G->ag->layouts[l - 1];
-        const Layout& cl = G->ag->layouts[l];
+        const std::vector<Layout>& lays = G->ag->layoutsFor(ORDER_STREAM);
+        const Layout& fl = (l == 0) ? G->A->layout(ORDER_STREAM) : lays[l - 1];
+        const Layout& cl = lays[l];
         const int nf = fl.n, nc = cl.n;
         DBuf<int> fperm, cperm;
         fperm.upload(fl.perm, c.stream);
@@ -1017,8 +1038,9 @@ int unap_hip_aggl_prolong_field(unap_hip_aggl G, int64_t l, const double* cf, do
         need(ff && cf, "null vector");
         ensureTransfers(G);
         Ctx& c = Ctx::get();
-        const Layout& fl = (l == 0) ? G->A->layout : G->ag->layouts[l - 1];
-        const Layout& cl = G->ag->layouts[l];
+        const std::vector<Layout>& lays = G->ag->layoutsFor(ORDER_STREAM);
+        const Layout& fl = (l == 0) ? G->A->layout(ORDER_STREAM) : lays[l - 1];
+        const Layout& cl = lays[l];
         const int nf = fl.n, nc = cl.n;
         DBuf<int> fperm, cperm;
         fperm.upload(fl.perm, c.stream);
@@ -1038,6 +1060,110 @@ int unap_hip_aggl_prolong_field(unap_hip_aggl G, int64_t l, const double* cf, do
             launchRows(BScatter<double, double>{fperm.get(), fi.get(), ff}, nf);
         }
         c.sync();
+    });
+}
+
+}  // extern "C"
+
+template <typename T>
+static void profileKernel(LduObj& L, int kernel, long long reps, double* avgMs, double* bytes) {
+    Ctx& c = Ctx::get();
+    const int order = (kernel == UNAP_HIP_KERNEL_GS || kernel == UNAP_HIP_KERNEL_DIC) ? ORDER_SWEEP : ORDER_STREAM;
+    DevMatrix<T>& M = Fine<T>::get(L, order);
+    DevMatrix<double>& M64 = L.dev64(order);
+    const Layout& lay = L.layout(order);
+    const int n = M.n;
+    const double s = sizeof(T), nn = n, f = L.host.nFaces();
+    DBuf<T> x(n), y(n), b(n), w(n);
+    DBuf<double> x64(n), r64(n), p64(n), w64(n);
+    launchRows(BFill<T>{x.get(), (T)0.5}, n);
+    launchRows(BFill<T>{b.get(), (T)1}, n);
+    launchRows(BFill<T>{w.get(), (T)0.25}, n);
+    for (auto* v : {&x64, &r64, &p64, &w64}) launchRows(BFill<double>{v->get(), 0.5}, n);
+    KState* est = c.scratchState.get();
+    kStateInit<<<1, 1, 0, c.stream>>>(est, 0, 0, 1, 0, nullptr, 0);
+    kLanczosMaxEig<<<1, 1, 0, c.stream>>>(est, 2, 1.1, 1.5);  // some finite table
+    DBuf<double> rD;
+    std::vector<double> rdh;
+    if (kernel == UNAP_HIP_KERNEL_DIC) {
+        if (!lay.sweepable) throw Error(-3, "DIC needs owner-sorted faces");
+        rdh = dicReciprocalD(L.host);
+        std::vector<double> rin(n);
+        for (int i = 0; i < n; ++i) rin[i] = rdh[lay.perm[i]];
+        rD.upload(rin, c.stream);
+        M64.buildValU(L.host, lay, c.stream);
+    }
+    const Skip none{nullptr, nullptr};
+    auto once = [&]() {
+        switch (kernel) {
+            case UNAP_HIP_KERNEL_AMUL:
+                launchRows(BAmul<T>{M.view(), x.get(), y.get()}, n);
+                break;
+            case UNAP_HIP_KERNEL_CHEBY:
+                launchRows(BCheby<T>{M.view(), x.get(), y.get(), b.get(), w.get(), est, 1}, n);
+                break;
+            case UNAP_HIP_KERNEL_GS:
+                gsSmooth<T>(M, x.get(), b.get(), 1, none);
+                break;
+            case UNAP_HIP_KERNEL_DIC:
+                luSweeps<double, double>(M64, true, w64.get(), r64.get(), rD.get(), none);
+                break;
+            case UNAP_HIP_KERNEL_RESIDUAL:
+                launchReduce<1>(BResidual<T>{M.view(), x.get(), b.get(), y.get(), nullptr}, n, M.iperm.get(),
+                                FinInitRes{est, 0});
+                break;
+            case UNAP_HIP_KERNEL_AXPY_NORM:
+                launchReduce<1>(BPcgXR{x64.get(), r64.get(), p64.get(), w64.get(), est}, n, M.iperm.get(),
+                                FinResNorm{est});
+                break;
+            case UNAP_HIP_KERNEL_DOT:
+                launchReduce<1>(BDot{x64.get(), p64.get()}, n, M.iperm.get(), FinPcgAlpha{est});
+                break;
+            case UNAP_HIP_KERNEL_COPY:
+                launchRows(BCopy<T, T>{x.get(), y.get()}, n);
+                break;
+            default:
+                throw Error(-1, "unknown kernel id");
+        }
+    };
+    const double s8 = (kernel == UNAP_HIP_KERNEL_DIC || kernel == UNAP_HIP_KERNEL_AXPY_NORM ||
+                       kernel == UNAP_HIP_KERNEL_DOT) ? 8.0 : s;
+    switch (kernel) {
+        case UNAP_HIP_KERNEL_AMUL: *bytes = (3 * s + 4) * nn + (2 * s + 8) * f; break;
+        case UNAP_HIP_KERNEL_CHEBY: *bytes = (6 * s + 4) * nn + (2 * s + 8) * f; break;
+        case UNAP_HIP_KERNEL_GS: *bytes = (4 * s + 4) * nn + (2 * s + 8) * f; break;
+        case UNAP_HIP_KERNEL_DIC: *bytes = (6 * s8 + 12) * nn + (2 * s8 + 8) * f; break;
+        case UNAP_HIP_KERNEL_RESIDUAL: *bytes = (4 * s + 4) * nn + (2 * s + 8) * f; break;
+        case UNAP_HIP_KERNEL_AXPY_NORM: *bytes = 6 * s8 * nn; break;
+        case UNAP_HIP_KERNEL_DOT: *bytes = 2 * s8 * nn; break;
+        case UNAP_HIP_KERNEL_COPY: *bytes = 2 * s * nn; break;
+    }
+    for (int i = 0; i < 3; ++i) once();
+    hipEvent_t e0, e1;
+    UH_CHECK(hipEventCreate(&e0));
+    UH_CHECK(hipEventCreate(&e1));
+    UH_CHECK(hipEventRecord(e0, c.stream));
+    for (long long i = 0; i < reps; ++i) once();
+    UH_CHECK(hipEventRecord(e1, c.stream));
+    UH_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    UH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avgMs = ms / (double)reps;
+}
+
+extern "C" {
+
+int unap_hip_profile_kernel(unap_hip_ldu A, int kernel, int precision, int64_t reps, double* avgMs, double* bytes) {
+    return guard([&] {
+        need(A && avgMs && bytes, "null argument");
+        need(reps >= 1, "reps must be positive");
+        need(precision == 64 || precision == 32, "precision must be 64 or 32");
+        if (precision == 64)
+            profileKernel<double>(A->o, kernel, reps, avgMs, bytes);
+        else
+            profileKernel<float>(A->o, kernel, reps, avgMs, bytes);
     });
 }
 
@@ -1098,11 +1224,14 @@ int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const do
         need(o.type != UNAP_HIP_SOLVER_MG || o.mgSet, "MG solver needs unap_hip_solver_set_mg first");
         Ctx& c = Ctx::get();
         if (where == UNAP_HIP_DEVICE) UH_CHECK(hipDeviceSynchronize());
-        DevMatrix<double>& M = L.dev64();
+        int order = ORDER_STREAM;
+        if (o.P) order = o.P->order;
+        else if (o.type == UNAP_HIP_SOLVER_MG && o.mg.smoother == UNAP_HIP_SMOOTHER_GS) order = ORDER_SWEEP;
+        DevMatrix<double>& M = L.dev64(order);
         const long long histCap = o.maxIter + 2;
         S->bufs.ensure(M.n, histCap, o.P && o.P->amg32);
         if (o.type == UNAP_HIP_SOLVER_MG && (!o.mgH || o.mgFor != &L)) {
-            auto ag = makeAggl(L.host, o.mgWeightsPtr, true, o.mg.maxLevels, o.mg.nCoarsest);
+            auto ag = agglFor(L, o.mgWeightsPtr, true, o.mg.maxLevels, o.mg.nCoarsest);
             o.mgH.reset(new Hierarchy<double>);
             o.mgH->build(L, ag, o.mg);
             o.mgFor = &L;
@@ -1117,7 +1246,7 @@ int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const do
         KState* st = S->bufs.st.get();
         kStateInit<<<1, 1, 0, c.stream>>>(st, o.tol, o.relTol, o.maxIter, o.minIter, S->bufs.hist.get(), histCap);
         UH_LAUNCHED();
-        Driver d(o, L, S->bufs);
+        Driver d(o, L, M, S->bufs);
         switch (o.type) {
             case UNAP_HIP_SOLVER_PCG: d.pcg(); break;
             case UNAP_HIP_SOLVER_PBICGSTAB: d.bicg(); break;

@@ -30,7 +30,7 @@ _intp = ctypes.POINTER(ctypes.c_int)
 class AmgParams(ctypes.Structure):
     _fields_ = [("smoother", _i64), ("n_cells_coarsest", _i64), ("max_levels", _i64),
                 ("n_vcycles", _i64), ("n_pre_sweeps", _i64), ("n_post_sweeps", _i64),
-                ("n_finest_sweeps", _i64), ("precision", _i64)]
+                ("n_finest_sweeps", _i64), ("precision", _i64), ("level_sweep_growth", _i64)]
 
 
 class Perf(ctypes.Structure):
@@ -81,12 +81,14 @@ SIGNATURES = {
     "unap_hip_solver_solve": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(Perf)]),
     "unap_hip_solver_history": (ctypes.c_int, [_vp, _f64p, _i64, _i64p]),
     "unap_hip_solver_last_ms": (ctypes.c_int, [_vp, _f64p]),
+    "unap_hip_profile_kernel": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i64, _f64p, _f64p]),
     "unap_hip_batch_create": (ctypes.c_int, [_i64, _i64p, _i64p, _i64p, _i64p, _f64p, _f64p, _f64p, _i32p,
                                              ctypes.POINTER(_vp)]),
     "unap_hip_batch_destroy": (ctypes.c_int, [_vp]),
     "unap_hip_batch_solve": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _i64, ctypes.c_double, ctypes.c_double,
                                             ctypes.POINTER(Perf)]),
     "unap_hip_batch_last_ms": (ctypes.c_int, [_vp, _f64p]),
+    "unap_hip_batch_config": (ctypes.c_int, [_vp, _i64p, _i64p]),
 }
 
 _lib = None
@@ -145,6 +147,76 @@ def device_info():
     return buf.value.decode(), int(cus.value)
 
 
+_rt = None
+
+
+def _runtime():
+    """The HIP runtime the library itself is linked against (same process
+    instance, resolved by soname after the library is loaded)."""
+    global _rt
+    if _rt is None:
+        lib()
+        R = ctypes.CDLL("libamdhip64.so.7")
+        R.hipMalloc.argtypes = [ctypes.POINTER(_vp), ctypes.c_size_t]
+        R.hipFree.argtypes = [_vp]
+        R.hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int]
+        R.hipMemset.argtypes = [_vp, ctypes.c_int, ctypes.c_size_t]
+        for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipMemset", "hipDeviceSynchronize"):
+            getattr(R, f).restype = ctypes.c_int
+        _rt = R
+    return _rt
+
+
+class DeviceArray:
+    """A float64 vector in device memory (hipMalloc), passed to the library
+    as a DEVICE pointer.  Plumbing for callers without their own allocator."""
+
+    def __init__(self, n, host=None):
+        R = _runtime()
+        self.n = int(n)
+        self.ptr = _vp()
+        if R.hipMalloc(ctypes.byref(self.ptr), max(8, 8 * self.n)) != 0:
+            raise RuntimeError(f"hipMalloc of {8 * self.n} bytes failed")
+        if host is None:
+            self.zero()
+        else:
+            self.upload(host)
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a, np.float64)
+        return cls(a.size, host=a)
+
+    def data_ptr(self):
+        return self.ptr.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        assert a.size == self.n
+        if _rt.hipMemcpy(self.ptr, a.ctypes.data_as(_vp), 8 * self.n, 1) != 0:
+            raise RuntimeError("hipMemcpy host->device failed")
+
+    def zero(self):
+        if _rt.hipMemset(self.ptr, 0, 8 * self.n) != 0 or _rt.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed")
+
+    def numpy(self):
+        out = np.empty(self.n)
+        if _rt.hipMemcpy(out.ctypes.data_as(_vp), self.ptr, 8 * self.n, 2) != 0:
+            raise RuntimeError("hipMemcpy device->host failed")
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value and _rt is not None:
+            _rt.hipFree(self.ptr)
+            self.ptr = None
+
+
+def synchronize():
+    if _runtime().hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+
+
 def amg_params(**kw):
     p = AmgParams()
     _check(lib().unap_hip_amg_params_default(ctypes.byref(p)))
@@ -195,6 +267,16 @@ class LduMatrix:
             return y
         _check(lib().unap_hip_ldu_spmv(self.h, xp, ctypes.c_void_p(out.data_ptr()), DEVICE))
         return out
+
+
+KERNELS = {"amul": 1, "cheby": 2, "gs": 3, "dic": 4, "residual": 5, "axpy_norm": 6, "dot": 7, "copy": 8}
+
+
+def profile_kernel(A, kernel, precision=64, reps=20):
+    """(mean ms per launch, algorithmic bytes per launch) of one kernel on A."""
+    ms, by = ctypes.c_double(), ctypes.c_double()
+    _check(lib().unap_hip_profile_kernel(A.h, KERNELS[kernel], precision, reps, ctypes.byref(ms), ctypes.byref(by)))
+    return ms.value, by.value
 
 
 class Precond:
@@ -411,3 +493,8 @@ class Batch:
         v = ctypes.c_double()
         _check(lib().unap_hip_batch_last_ms(self.h, ctypes.byref(v)))
         return v.value
+
+    def config(self):
+        bs, wg = _i64(), _i64()
+        _check(lib().unap_hip_batch_config(self.h, ctypes.byref(bs), ctypes.byref(wg)))
+        return bs.value, wg.value
