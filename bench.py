@@ -47,6 +47,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-size", type=int, default=None, help="edge length of the CPU sample (default: --size)")
     ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch of the dominant kernel")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-systems section")
+    ap.add_argument("--batch-launches", type=int, default=20)
+    ap.add_argument("--batch-systems", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
@@ -65,15 +68,15 @@ def main():
 
     t0 = time.time()
     A = hip.LduMatrix.from_system(s)
-    t_layout = time.time() - t0
+    t_create = time.time() - t0
     info = A.info()
-    log(f"layout {t_layout:.1f}s", info)
+    log(f"ldu_create {t_create:.1f}s", info)
 
     common = dict(smoother=2, n_cells_coarsest=args.n_coarsest, n_vcycles=1, n_post_sweeps=4, n_finest_sweeps=4)
     t0 = time.time()
     P32 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, level_sweep_growth=0, **common))
     t_setup32 = time.time() - t0
-    log(f"mixed hierarchy setup {t_setup32:.1f}s (includes agglomeration)")
+    log(f"mixed hierarchy setup {t_setup32:.1f}s (device layout, agglomeration, coarse layouts, estimates)")
     t0 = time.time()
     P64 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=64, level_sweep_growth=1, **common))
     t_setup64 = time.time() - t0
@@ -147,6 +150,11 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_size or N, p64["nIterations"], args.n_coarsest)
 
+    batch = None
+    if not args.no_batch:
+        del P32, P64, S32, S64, Sref, A, x, b
+        batch = batch_bench(args.batch_systems, args.batch_launches, not args.no_cpu)
+
     t32 = float(np.median(times32))
     out = {
         "metric": METRIC, "value": round(t32, 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
@@ -161,11 +169,99 @@ def main():
         "fp64_drop_in": {"s": round(t64, 5), "iterations": p64["nIterations"], "converged": p64["converged"],
                          "err_vs_fp64_1e-12": err64},
         "speedup_mixed_vs_fp64": round(t64 / t32, 3),
-        "setup_s": {"generate": round(t_gen, 2), "layout": round(t_layout, 2), "mixed_hierarchy": round(t_setup32, 2),
-                    "fp64_hierarchy": round(t_setup64, 2)},
+        "setup_s": {"generate": round(t_gen, 2), "ldu_create": round(t_create, 2),
+                    "mixed_hierarchy": round(t_setup32, 2), "fp64_hierarchy": round(t_setup64, 2)},
         "device": dev_name, "roofline": roofline, "kernels": kernels, "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
+
+
+def _ldu_residual(s, x):
+    """||b - A x|| / ||b|| with A applied from the LDU arrays (host numpy)."""
+    y = s.diag * x
+    np.add.at(y, s.l, s.upper * x[s.u])
+    np.add.at(y, s.u, s.lower_or_upper() * x[s.l])
+    return float(np.linalg.norm(s.b - y) / np.linalg.norm(s.b))
+
+
+def batch_bench(nsys, launches, with_cpu):
+    """Workload C5: nsys independent permuted LDU box systems (edges 8..48,
+    60 % SPD Poisson by PCG + DIC, 40 % convection-diffusion by PBiCGStab +
+    DILU, FP32 preconditioners in FP64 Krylov loops) in one persistent launch
+    per solve; systems/s and launch-latency percentiles over `launches`."""
+    from unap_amd import hip, meshgen
+    t0 = time.time()
+    specs = meshgen.batch_specs(nsys=nsys)
+    systems = [meshgen.batch_system(sp) for sp in specs]
+    solvers = ["PCG" if sp[3] == meshgen.POISSON_LOGUNIF else "PBiCGStab" for sp in specs]
+    t_gen = time.time() - t0
+    t0 = time.time()
+    B = hip.Batch(systems, solvers)
+    t_create = time.time() - t0
+    ncells = int(B.cell_off[-1])
+    log(f"batch: {nsys} systems, {ncells} cells, generated {t_gen:.1f}s, created {t_create:.1f}s")
+    bh = np.concatenate([s.b for s in systems])
+    b = hip.DeviceArray.from_numpy(bh)
+    x = hip.DeviceArray(ncells)
+    t0 = time.time()
+    _, perfs = B.solve(b, x=x, rel_tol=1e-8, max_iter=1000)  # first solve autotunes
+    t_tune = time.time() - t0
+    bs, wg = B.config()
+    log(f"batch autotune {t_tune:.1f}s -> block {bs}, {wg} workgroups/CU")
+    lat = []
+    for _ in range(launches):
+        x.zero()
+        _, perfs = B.solve(b, x=x, rel_tol=1e-8, max_iter=1000)
+        lat.append(B.last_ms())
+    lat = np.array(lat)
+    xh = x.numpy()
+    off = B.cell_off
+    res = [_ldu_residual(s, xh[off[i]:off[i + 1]]) for i, s in enumerate(systems)]
+    its = np.array([p["nIterations"] for p in perfs])
+    conv = all(p["converged"] for p in perfs)
+    p50 = float(np.percentile(lat, 50))
+    log(f"batch launch ms p50 {p50:.2f} p90 {np.percentile(lat, 90):.2f} p99 {np.percentile(lat, 99):.2f}; "
+        f"{nsys / (p50 * 1e-3):.0f} systems/s; all converged {conv}; max true rel. residual {max(res):.2e}")
+    out = {"workload": f"C5: {nsys} permuted LDU boxes, edges 8..48 ({ncells} cells), 60% PCG+DIC / "
+                       "40% PBiCGStab+DILU, FP32 preconditioners, FP64 Krylov, rel.tol 1e-8, one launch",
+           "systems_per_s": round(nsys / (p50 * 1e-3), 1), "launches": launches,
+           "latency_ms": {"p50": round(p50, 3), "p90": round(float(np.percentile(lat, 90)), 3),
+                          "p99": round(float(np.percentile(lat, 99)), 3), "min": round(float(lat.min()), 3)},
+           "all_converged": conv, "max_true_rel_residual": max(res),
+           "iterations": {"min": int(its.min()), "median": float(np.median(its)), "max": int(its.max())},
+           "autotuned": {"block": bs, "workgroups_per_cu": wg, "tune_s": round(t_tune, 2)},
+           "setup_s": {"generate": round(t_gen, 2), "create": round(t_create, 2)}}
+    if with_cpu:
+        out["cpu_baseline"] = batch_cpu_baseline(systems, solvers, p50)
+    return out
+
+
+def batch_cpu_baseline(systems, solvers, gpu_ms, stride=16):
+    """The same systems solved one after another by the CPU path (FP64 DIC /
+    DILU as the reference runs them) on one core: every `stride`-th system
+    timed (preconditioner setup + solve), scaled by `stride`."""
+    from oracle import oracle as O
+    path = O.reference()
+    kind = "reference"
+    if path is None:
+        path = O.oracle()
+        kind = "port"
+    idx = list(range(0, len(systems), stride))
+    t = 0.0
+    for k, i in enumerate(idx):
+        s = systems[i]
+        M = path.matrix(s)
+        pre = "DIC" if solvers[i] == "PCG" else "DILU"
+        _, t_setup, t_solve, _ = O.timed_solve(path, M, solvers[i], pre, s.b, 1000, 1, 1e-8)
+        t += t_setup + t_solve
+        if k % 8 == 7:
+            log(f"batch cpu: {k + 1}/{len(idx)} systems, {t:.2f}s")
+    est = t * len(systems) / len(idx)
+    log(f"batch cpu ({kind}): {len(idx)} systems in {t:.2f}s -> {est:.1f}s for {len(systems)}")
+    return {"value": round(len(systems) / est, 2), "unit": "systems/s", "cores": 1, "kind": kind,
+            "sample": f"every {stride}th system ({len(idx)} of {len(systems)}) solved in FP64 on one core "
+                      f"(preconditioner setup + solve, {t:.2f}s), scaled by {len(systems) / len(idx):.0f}"}
 
 
 def cpu_baseline(N, iters_fp64, n_coarsest):

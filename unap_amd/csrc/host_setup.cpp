@@ -181,6 +181,23 @@ void levels(const HostLdu& A, const RowFaces& R, bool sweepable, std::vector<int
     }
 }
 
+// SELL-C-sigma: inside consecutive windows of sigma rows of perm[b, e) the
+// rows are stably sorted by descending length, so each 64-row slice holds
+// rows of similar length.  Off by default: on the agglomerated levels of a
+// 256^3 Poisson matrix (6..40 entries a row) sorting cuts padded entries from
+// 1.46x to 1.04x (sigma 1024) or 1.14x (256), but spreading a slice's rows
+// over the window scatters its x gathers, and the FP32 Amul over levels 1-8
+// took 0.80 ms (1024) and 0.62 ms (256) against 0.55 ms unsorted.
+constexpr int kSigma = 0;
+void sortWindows(std::vector<int>& perm, const RowFaces& R, int b, int e, int sigma) {
+    if (sigma <= 1) return;
+    auto len = [&](int c) { return R.rs[c + 1] - R.rs[c]; };
+    for (int s = b; s < e; s += sigma) {
+        const int t = std::min(e, s + sigma);
+        std::stable_sort(perm.begin() + s, perm.begin() + t, [&](int x, int y) { return len(x) > len(y); });
+    }
+}
+
 }  // namespace
 
 int forcedOrderMode() {
@@ -203,7 +220,8 @@ void levelCounts(const HostLdu& A, int& nFwd, int& nBwd) {
     levels(A, R, true, lev, nFwd, blev, nBwd);
 }
 
-Layout buildLayout(const HostLdu& A, int orderMode) {
+Layout buildLayout(const HostLdu& A, int orderMode, int sigma) {
+    if (sigma < 0) sigma = kSigma;
     const int forced = forcedOrderMode();
     if (forced >= 0) orderMode = forced;
     const RowFaces R = rowFaces(A);
@@ -218,11 +236,14 @@ Layout buildLayout(const HostLdu& A, int orderMode) {
         const std::vector<int> bfs = bfsOrder(A, R);
         if (orderMode == 1) {
             perm = bfs;
+            sortWindows(perm, R, 0, A.n, sigma);
         } else {  // forward level major, breadth-first minor
             std::vector<int> start(nlev + 1, 0);
             for (int c = 0; c < A.n; ++c) start[lev[c] + 1]++;
             for (int k = 0; k < nlev; ++k) start[k + 1] += start[k];
-            for (int i = 0; i < A.n; ++i) perm[start[lev[bfs[i]]]++] = bfs[i];
+            std::vector<int> at(start.begin(), start.end() - 1);
+            for (int i = 0; i < A.n; ++i) perm[at[lev[bfs[i]]]++] = bfs[i];
+            for (int k = 0; k < nlev; ++k) sortWindows(perm, R, start[k], start[k + 1], sigma);
         }
     }
     return finishLayout(A, R, std::move(perm), lev, nlev, blev, nblev, sweepable);

@@ -53,7 +53,9 @@ struct Layout {
 // read contiguous rows); 1: breadth-first only (Amul and Chebyshev stream at
 // twice the level-major rate; sweeps gather their rows through fwdRows);
 // 2: the caller's order.
-Layout buildLayout(const HostLdu& A, int orderMode);
+// sigma: rows are sorted by length inside windows of sigma rows (0: off,
+// the default).
+Layout buildLayout(const HostLdu& A, int orderMode, int sigma = -1);
 // Order forced for every layout by the environment variable
 // UNAP_HIP_CELL_ORDER ("level" 0, "bfs" 1, "natural" 2), or -1 when unset.
 int forcedOrderMode();

@@ -473,15 +473,50 @@ struct RowEnt {
 
 // Gauss-Seidel row (lduGaussSeidelSmoother.cpp:53-78): b minus the lower-side
 // terms (new x) then the upper-side terms (old x), then / diag.
+// acc -= c_k * x[col_k] over entries k = k0, k0 + step, ... (kEnd excluded),
+// loads issued kUnroll at a time, subtractions in entry order.  c_k is the
+// stored coefficient times `scale` when Scaled (DIC/DILU: rD * coef), the
+// coefficient itself otherwise.
+template <bool Scaled, typename T, typename TV, typename TX>
+__device__ __forceinline__ TV rowSub(const DMat<T>& A, const RowEnt<T>& e, int k0, int kEnd, int step, TV acc,
+                                     TV scale, const TX* x) {
+    int k = k0;
+    const int n = (kEnd - k0) * step;  // entries left (step = +-1)
+    int left = n;
+    for (; left >= kUnroll; left -= kUnroll, k += kUnroll * step) {
+        int c[kUnroll];
+        T v[kUnroll];
+        TX xv[kUnroll];
+#pragma unroll
+        for (int j = 0; j < kUnroll; ++j) {
+            const long long p = e.at(k + j * step);
+            c[j] = A.col[p];
+            v[j] = A.val[p];
+        }
+#pragma unroll
+        for (int j = 0; j < kUnroll; ++j) xv[j] = x[c[j]];
+#pragma unroll
+        for (int j = 0; j < kUnroll; ++j) {
+            if (Scaled)
+                acc = acc - (scale * (TV)v[j]) * (TV)xv[j];
+            else
+                acc = acc - (TV)v[j] * (TV)xv[j];
+        }
+    }
+    for (; left > 0; --left, k += step) {
+        const long long p = e.at(k);
+        if (Scaled)
+            acc = acc - (scale * (TV)A.val[p]) * (TV)x[A.col[p]];
+        else
+            acc = acc - (TV)A.val[p] * (TV)x[A.col[p]];
+    }
+    return acc;
+}
+
 template <typename T>
 __device__ __forceinline__ void gsRow(const DMat<T>& A, int r, T* x, const T* b) {
     RowEnt<T> e(A, r);
-    T acc = b[r];
-    const int len = e.len();
-    for (int k = 0; k < len; ++k) {
-        const long long p = e.at(k);
-        acc = acc - A.val[p] * x[A.col[p]];
-    }
+    T acc = rowSub<false, T, T, T>(A, e, 0, e.len(), 1, b[r], (T)1, x);
     acc = acc / A.diag[r];
     x[r] = acc;
 }
@@ -491,26 +526,14 @@ template <typename T, typename TV>
 __device__ __forceinline__ void fwdRow(const DMat<T>& A, int r, TV* w, const TV* rr, const TV* rD) {
     RowEnt<T> e(A, r);
     const TV rd = rD[r];
-    TV acc = rd * rr[r];
-    const int mid = e.mid();
-    for (int k = 0; k < mid; ++k) {
-        const long long p = e.at(k);
-        acc = acc - (rd * (TV)A.val[p]) * w[A.col[p]];
-    }
-    w[r] = acc;
+    w[r] = rowSub<true, T, TV, TV>(A, e, 0, e.mid(), 1, rd * rr[r], rd, w);
 }
 // backward row (lduDICPrecond.cpp:151-154): owner faces in descending order
 template <typename T, typename TV>
 __device__ __forceinline__ void bwdRow(const DMat<T>& A, int r, TV* w, const TV* rD) {
     RowEnt<T> e(A, r);
     const TV rd = rD[r];
-    TV acc = w[r];
-    const int mid = e.mid();
-    for (int k = e.len() - 1; k >= mid; --k) {
-        const long long p = e.at(k);
-        acc = acc - (rd * (TV)A.val[p]) * w[A.col[p]];
-    }
-    w[r] = acc;
+    w[r] = rowSub<true, T, TV, TV>(A, e, e.len() - 1, e.mid() - 1, -1, w[r], rd, w);
 }
 
 enum SweepMode { SWEEP_GS = 0, SWEEP_FWD = 1, SWEEP_BWD = 2 };

@@ -327,12 +327,12 @@ void Hierarchy<T>::precondition(const double* r, T* w, Skip sk) {
     launchRows(BFill<T>{w, (T)0}, n, sk);
     const bool fp32 = sizeof(T) == 4;
     T* source = fp32 ? src32.get() : reinterpret_cast<T*>(const_cast<double*>(r));
+    // FP32 hierarchy: the FP64 residual is cast once, coalesced, and the
+    // restriction gathers FP32 values
+    if (fp32) launchRows(BCopy<double, T>{r, src32.get()}, n, sk);
     for (int cyc = 0; cyc < prm.nVcycles; ++cyc) {
         if (cyc == 0) {
-            if (fp32)
-                vcycle(w, source, src32.get(), r, sk);
-            else
-                vcycle(w, source, source, nullptr, sk);
+            vcycle(w, source, source, nullptr, sk);
         } else {
             launchRows(BResidNoNorm<T>{fine->view(), w, source, fw.b.get(), fres.get()}, n, sk);
             vcycle(w, source, fres.get(), nullptr, sk);
