@@ -50,6 +50,13 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-systems section")
     ap.add_argument("--batch-launches", type=int, default=20)
     ap.add_argument("--batch-systems", type=int, default=512)
+    ap.add_argument("--kernel-only", default=None,
+                    help="only time one finest-level kernel (amul, cheby, gs, dic, residual, axpy_norm, dot, copy) "
+                         "on the 256^3 matrix, for hardware-counter runs")
+    ap.add_argument("--precision", type=int, default=32, help="precision of --kernel-only")
+    ap.add_argument("--reps", type=int, default=5, help="launches of --kernel-only")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="stop after the timed mixed solves (for kernel traces of exactly those solves)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
@@ -61,6 +68,13 @@ def main():
     dev_name, cus = hip.device_info()
     log("device:", dev_name, cus, "CUs")
     N = args.size
+    if args.kernel_only:
+        s = meshgen.config_system(4, N)
+        A = hip.LduMatrix.from_system(s)
+        ms, by = hip.profile_kernel(A, args.kernel_only, args.precision, reps=args.reps)
+        print(json.dumps({"kernel": args.kernel_only, "precision": args.precision, "n": s.n, "n_faces": s.nfaces,
+                          "reps": args.reps, "ms": ms, "alg_bytes": by, "GBps": by / (ms * 1e-3) / 1e9}), flush=True)
+        return
     t0 = time.time()
     s = meshgen.config_system(4, N)
     t_gen = time.time() - t0
@@ -95,6 +109,7 @@ def main():
         hip.synchronize()
         return time.perf_counter() - t, perf
 
+    levels32 = P32.amg_levels()
     for _ in range(args.warmup):
         solve(S32)
     log("warmup done")
@@ -105,6 +120,9 @@ def main():
         log(f"mixed step {k}: {dt * 1e3:.1f} ms, {p32['nIterations']} iterations, converged={p32['converged']}")
     x32 = x.numpy()
     ms_dev32 = S32.last_ms()
+    if args.mixed_only:
+        print(json.dumps({"mixed_s": times32, "iterations": p32["nIterations"], "hierarchy": levels32}), flush=True)
+        return
 
     times64 = []
     for k in range(max(1, args.fp64_steps)):
@@ -164,6 +182,7 @@ def main():
                    "n_cells": s.n, "n_faces": s.nfaces, "mixed": "FCG(FP64) + FP32 AMG V(0,4) Chebyshev(4), "
                    "coarsest dense FP64", "fp64": "PCG + FP64 AMG V(0,4+level) Chebyshev(4), coarsest PCG+DIC 1e-2",
                    "n_cells_coarsest": args.n_coarsest},
+        "hierarchy_levels": levels32,
         "mixed": {"s": round(t32, 5), "device_ms": round(ms_dev32, 3), "iterations": p32["nIterations"],
                   "converged": p32["converged"], "err_vs_fp64_1e-12": err32, "true_rel_residual": true_res32},
         "fp64_drop_in": {"s": round(t64, 5), "iterations": p64["nIterations"], "converged": p64["converged"],

@@ -121,6 +121,12 @@ int unap_hip_precond_destroy(unap_hip_precond P);
 int unap_hip_precond_apply(unap_hip_precond P, const double* r, double* w, int where);
 /* rD() (unapMatrix.hpp:343): reciprocal (preconditioned) diagonal, host copy. */
 int unap_hip_precond_rd(unap_hip_precond P, double* rD);
+/* Sizes of an AMG preconditioner's hierarchy: level 0 is the matrix itself,
+   level l >= 1 the coarse level of agglomeration step l (the reference's
+   coarseMatrixLevels(l - 1), lduAgglomeration.hpp:83-92).  Writes up to cap
+   entries; *n_levels receives the total count. */
+int unap_hip_precond_amg_levels(unap_hip_precond P, int64_t cap, int64_t* n_levels, int64_t* n_cells,
+                                int64_t* n_faces);
 
 /* ----------------------------------------------------------------- smoothers
  * Replaces matrix::smoother::smooth(x, A, b, nSweeps) and init()

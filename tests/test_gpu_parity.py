@@ -219,6 +219,18 @@ def test_amg_pcg_parity(oracle, order, sm, nv):
     _check_solve(S.solve(A, s.b), ref, ("AMG-PCG", sm, nv), order)
 
 
+@pytest.mark.parametrize("pre,mg", [("DIC", None), ("AMG", [1, 20, 1, 0, 2, 2])])
+def test_fcg_fp64_parity(oracle, order, pre, mg):
+    # the flexible outer loop in FP64 against the oracle's FCG
+    s = _sys(2, 16)
+    A = hip.LduMatrix.from_system(s)
+    amg = None if mg is None else hip.amg_params(smoother=1, n_cells_coarsest=20, n_vcycles=1)
+    P = hip.Precond(A, pre, amg=amg)
+    S = hip.Solver("FCG", P, rel_tol=1e-8, max_iter=300)
+    ref = oracle.matrix(s).solve("FCG", pre, s.b, rel_tol=1e-8, max_iter=300, mg=mg)
+    _check_solve(S.solve(A, s.b), ref, ("FCG", pre), order)
+
+
 def test_mixed_fcg_accuracy(oracle):
     s = _sys(4, 20)
     A = hip.LduMatrix.from_system(s)

@@ -112,14 +112,15 @@ Layout finishLayout(const HostLdu& A, const RowFaces& R, std::vector<int> perm, 
     if (!contiguous) L.fwdRows = std::move(fr);
     L.bwdRows = groupByLevel(bi, nblev, L.bwdStart);
 
-    // SELL-64: slice s holds internal rows 64s..64s+63, entry k of a row at
-    // sliceStart[s] + 64k + (row & 63).
-    L.nslices = (n + 63) / 64;
+    // SELL-32 (kSliceRows): slice s holds internal rows 32s..32s+31, entry k
+    // of a row at sliceStart[s] + 32k + (row & 31).
+    const int C = kSliceRows;
+    L.nslices = (n + C - 1) / C;
     L.sliceStart.assign(L.nslices + 1, 0);
     L.rowInfo.assign(n, 0);
     for (int s = 0; s < L.nslices; ++s) {
         int mx = 0;
-        for (int r = 64 * s; r < std::min(n, 64 * s + 64); ++r) {
+        for (int r = C * s; r < std::min(n, C * s + C); ++r) {
             const int c = L.perm[r];
             const int len = R.rs[c + 1] - R.rs[c];
             if (len > 0xffff) throw std::runtime_error("row longer than 65535 entries");
@@ -127,19 +128,19 @@ Layout finishLayout(const HostLdu& A, const RowFaces& R, std::vector<int> perm, 
             L.rowInfo[r] = (uint32_t)len | ((uint32_t)R.nLowerSide[c] << 16);
         }
         L.maxRowLen = std::max(L.maxRowLen, mx);
-        L.sliceStart[s + 1] = L.sliceStart[s] + 64LL * mx;
+        L.sliceStart[s + 1] = L.sliceStart[s] + (long long)C * mx;
     }
     L.nnzPadded = L.sliceStart[L.nslices];
     L.col.assign(L.nnzPadded, 0);
     L.src.assign(L.nnzPadded, -1);
     for (int s = 0; s < L.nslices; ++s) {
-        const int mx = (int)((L.sliceStart[s + 1] - L.sliceStart[s]) / 64);
-        for (int lane = 0; lane < 64; ++lane) {
-            const int r = 64 * s + lane;
+        const int mx = (int)((L.sliceStart[s + 1] - L.sliceStart[s]) / C);
+        for (int lane = 0; lane < C; ++lane) {
+            const int r = C * s + lane;
             const int c = r < n ? L.perm[r] : -1;
             const int len = r < n ? R.rs[c + 1] - R.rs[c] : 0;
             for (int k = 0; k < mx; ++k) {
-                const long long p = L.sliceStart[s] + 64LL * k + lane;
+                const long long p = L.sliceStart[s] + (long long)C * k + lane;
                 if (k < len) {
                     const int e = R.ent[R.rs[c] + k];
                     const int f = e >> 1;
@@ -182,13 +183,13 @@ void levels(const HostLdu& A, const RowFaces& R, bool sweepable, std::vector<int
 }
 
 // SELL-C-sigma: inside consecutive windows of sigma rows of perm[b, e) the
-// rows are stably sorted by descending length, so each 64-row slice holds
-// rows of similar length.  Off by default: on the agglomerated levels of a
-// 256^3 Poisson matrix (6..40 entries a row) sorting cuts padded entries from
-// 1.46x to 1.04x (sigma 1024) or 1.14x (256), but spreading a slice's rows
-// over the window scatters its x gathers, and the FP32 Amul over levels 1-8
-// took 0.80 ms (1024) and 0.62 ms (256) against 0.55 ms unsorted.
-constexpr int kSigma = 0;
+// rows are stably sorted by descending length, so each slice holds rows of
+// similar length.  The window is one wavefront (64 rows): wider windows cut
+// padding further but move rows between wavefronts and scatter their x
+// gathers (on the agglomerated levels of a 256^3 Poisson matrix, 64-row
+// slices sorted over 1024-row windows padded 1.04x yet ran the FP32 Amul over
+// levels 1-8 in 0.80 ms against 0.55 ms unsorted).
+constexpr int kSigma = kSortWindow;
 void sortWindows(std::vector<int>& perm, const RowFaces& R, int b, int e, int sigma) {
     if (sigma <= 1) return;
     auto len = [&](int c) { return R.rs[c + 1] - R.rs[c]; };

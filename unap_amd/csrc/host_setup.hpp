@@ -1,5 +1,5 @@
 // host_setup.hpp -- host-side setup of the solve path: device layout of an LDU
-// matrix (cell ordering, SELL-64 slices, level schedules), pairwise
+// matrix (cell ordering, SELL-32 slices, level schedules), pairwise
 // agglomeration, coarse addressing and Galerkin coarse coefficients.
 //
 // Numbering: "external" ids are the caller's (for coarse levels: the coarse ids
@@ -15,6 +15,19 @@
 #include <vector>
 
 namespace unap_hip {
+
+// SELL slice height: slice s holds internal rows 32s .. 32s+31, entry k of a
+// row at sliceStart[s] + 32k + (row & 31).  A 64-lane wavefront covers two
+// slices; its rows are sorted by length (stably, inside each aligned group of
+// 64 rows), so the two slices pad less.  A wavefront's k-th load still
+// touches two 128 B lines (FP32) or four (FP64), as with 64-row slices,
+// while on agglomerated coarse levels (6..40 entries a row) padded entries
+// fall from 1.46-1.51x to 1.22-1.25x; the rows a wavefront gathers for are
+// unchanged.  (16-row slices pad 1.10x but touch twice the lines per load:
+// the finest FP32 Amul fell from 4.6 to 4.2 TB/s.)
+constexpr int kSliceShift = 5;
+constexpr int kSliceRows = 1 << kSliceShift;
+constexpr int kSortWindow = 64;
 
 // An LDU matrix in external numbering (the lduMatrix data of the reference,
 // src/matrix/lduMatrix/lduMatrix.hpp:21-42, with 32-bit addressing).
@@ -53,8 +66,8 @@ struct Layout {
 // read contiguous rows); 1: breadth-first only (Amul and Chebyshev stream at
 // twice the level-major rate; sweeps gather their rows through fwdRows);
 // 2: the caller's order.
-// sigma: rows are sorted by length inside windows of sigma rows (0: off,
-// the default).
+// sigma: rows are sorted by length inside windows of sigma rows (default
+// kSortWindow, one wavefront; 0: off).
 Layout buildLayout(const HostLdu& A, int orderMode, int sigma = -1);
 // Order forced for every layout by the environment variable
 // UNAP_HIP_CELL_ORDER ("level" 0, "bfs" 1, "natural" 2), or -1 when unset.

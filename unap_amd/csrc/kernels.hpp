@@ -1,9 +1,9 @@
 // kernels.hpp -- CDNA4 (gfx950) kernels of the LDU solve path.
 //
-// Storage: every matrix lives in SELL-64 slices in the internal cell order (see
-// host_setup.hpp).  One thread owns one row; a wavefront owns one 64-row slice,
-// so the k-th coefficient/column load of a wavefront is one contiguous 512 B
-// (FP64) or 256 B (FP32) burst.  Rows accumulate strictly in the reference's
+// Storage: every matrix lives in SELL-32 slices in the internal cell order (see
+// host_setup.hpp).  One thread owns one row; a wavefront owns two 32-row
+// slices, so the k-th coefficient/column load of a wavefront is two
+// contiguous 256 B (FP64) or 128 B (FP32) segments.  Rows accumulate strictly in the reference's
 // order (diagonal first, then faces in ascending face id) with separately
 // rounded multiplies and adds (built with -ffp-contract=off), which makes the
 // FP64 Amul, sweeps and transfers bit-identical to the serial x86 loops.
@@ -20,6 +20,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "host_setup.hpp"  // kSliceShift
 
 namespace unap_hip {
 
@@ -71,11 +73,11 @@ struct Skip {
 constexpr int kUnroll = 4;
 template <typename T, typename TA, typename TX>
 __device__ __forceinline__ TA rowAmul(const DMat<T>& A, int r, const TX* __restrict__ x) {
-    const int s = r >> 6;
+    const int s = r >> kSliceShift;
     const long long s0 = A.sliceStart[s];
-    const int slen = (int)((A.sliceStart[s + 1] - s0) >> 6);
+    const int slen = (int)((A.sliceStart[s + 1] - s0) >> kSliceShift);
     const int len = (int)(A.rowInfo[r] & 0xffffu);
-    const long long base = s0 + (r & 63);
+    const long long base = s0 + (r & (kSliceRows - 1));
     TA acc = (TA)0 + (TA)A.diag[r] * (TA)x[r];
     int k = 0;
     for (; k + kUnroll <= slen; k += kUnroll) {
@@ -84,7 +86,7 @@ __device__ __forceinline__ TA rowAmul(const DMat<T>& A, int r, const TX* __restr
         TX xv[kUnroll];
 #pragma unroll
         for (int j = 0; j < kUnroll; ++j) {
-            const long long p = base + ((long long)(k + j) << 6);
+            const long long p = base + ((long long)(k + j) << kSliceShift);
             c[j] = A.col[p];
             v[j] = A.val[p];
         }
@@ -95,7 +97,7 @@ __device__ __forceinline__ TA rowAmul(const DMat<T>& A, int r, const TX* __restr
             if (k + j < len) acc = acc + (TA)v[j] * (TA)xv[j];
     }
     for (; k < slen; ++k) {
-        const long long p = base + ((long long)k << 6);
+        const long long p = base + ((long long)k << kSliceShift);
         const TA v = (TA)A.val[p];
         const TA xv = (TA)x[A.col[p]];
         if (k < len) acc = acc + v * xv;

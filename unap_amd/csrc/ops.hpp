@@ -463,12 +463,12 @@ struct RowEnt {
     long long base;
     uint32_t info;
     __device__ RowEnt(const DMat<T>& A, int r) {
-        base = A.sliceStart[r >> 6] + (r & 63);
+        base = A.sliceStart[r >> kSliceShift] + (r & (kSliceRows - 1));
         info = A.rowInfo[r];
     }
     __device__ int len() const { return (int)(info & 0xffffu); }
     __device__ int mid() const { return (int)(info >> 16); }
-    __device__ long long at(int k) const { return base + ((long long)k << 6); }
+    __device__ long long at(int k) const { return base + ((long long)k << kSliceShift); }
 };
 
 // Gauss-Seidel row (lduGaussSeidelSmoother.cpp:53-78): b minus the lower-side

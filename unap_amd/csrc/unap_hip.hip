@@ -845,6 +845,23 @@ int unap_hip_precond_rd(unap_hip_precond P, double* rD) {
     });
 }
 
+int unap_hip_precond_amg_levels(unap_hip_precond P, int64_t cap, int64_t* nLevels, int64_t* nCells,
+                                int64_t* nFaces) {
+    return guard([&] {
+        need(P != nullptr && nLevels != nullptr, "null argument");
+        const PrecondObj& o = P->o;
+        const AgglHost* ag = o.amg32 ? o.amg32->ag.get() : (o.amg64 ? o.amg64->ag.get() : nullptr);
+        if (!ag) throw Error(UNAP_HIP_ERR_STATE, "not an AMG preconditioner");
+        const int64_t L = 1 + (int64_t)ag->levels.size();
+        *nLevels = L;
+        for (int64_t l = 0; l < std::min<int64_t>(L, cap); ++l) {
+            const HostLdu& H = (l == 0) ? o.A->host : ag->levels[l - 1].A;
+            if (nCells) nCells[l] = H.n;
+            if (nFaces) nFaces[l] = H.nFaces();
+        }
+    });
+}
+
 int unap_hip_smoother_create(int type, unap_hip_smoother* out) {
     return guard([&] {
         need(out != nullptr, "out is null");

@@ -58,6 +58,7 @@ SIGNATURES = {
     "unap_hip_precond_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(AmgParams), _f64p, ctypes.POINTER(_vp)]),
     "unap_hip_precond_destroy": (ctypes.c_int, [_vp]),
     "unap_hip_precond_apply": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "unap_hip_precond_amg_levels": (ctypes.c_int, [_vp, _i64, _i64p, _i64p, _i64p]),
     "unap_hip_precond_rd": (ctypes.c_int, [_vp, _f64p]),
     "unap_hip_smoother_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     "unap_hip_smoother_destroy": (ctypes.c_int, [_vp]),
@@ -303,6 +304,15 @@ class Precond:
         out = np.empty(self.A.n)
         _check(lib().unap_hip_precond_rd(self.h, _p(out)))
         return out
+
+    def amg_levels(self):
+        """[(n_cells, n_faces)] of the AMG hierarchy, level 0 = the matrix."""
+        L = _i64()
+        _check(lib().unap_hip_precond_amg_levels(self.h, 0, ctypes.byref(L), None, None))
+        n = np.zeros(L.value, np.int64)
+        f = np.zeros(L.value, np.int64)
+        _check(lib().unap_hip_precond_amg_levels(self.h, L.value, ctypes.byref(L), _p(n, _i64p), _p(f, _i64p)))
+        return [(int(a), int(b)) for a, b in zip(n, f)]
 
 
 class Smoother:
