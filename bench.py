@@ -46,7 +46,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-size", type=int, default=None, help="edge length of the CPU sample (default: --size)")
-    ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch of the dominant kernel")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_256.json"),
+                    help="JSON with the HBM bytes per launch of the dominant kernel, measured by rocprofv3 --pmc")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-systems section")
     ap.add_argument("--batch-launches", type=int, default=20)
     ap.add_argument("--batch-systems", type=int, default=512)
@@ -156,11 +157,13 @@ def main():
         log(f"kernel {name} fp{prec}: {ms:.3f} ms, {gbs:.0f} GB/s ({gbs / PEAK_HBM_GBS:.0%} of 8 TB/s)")
     dom = kernels["cheby_fp32"]
     traffic = None
-    if args.traffic and os.path.exists(args.traffic):
-        traffic = json.load(open(args.traffic)).get("cheby_fp32_bytes")
+    if args.traffic and os.path.exists(args.traffic) and N == 256:
+        traffic = json.load(open(args.traffic))["cheby_fp32"]["traffic_bytes"]
     roofline = {"kernel": "Chebyshev sweep, FP32, finest level (fused Amul + update)", "bound": "hbm",
                 "achieved": dom["GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": dom["frac"],
-                "alg_bytes_per_launch": dom["alg_bytes"], "traffic": traffic}
+                "alg_bytes_per_launch": dom["alg_bytes"], "traffic": traffic,
+                "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of bench.py --kernel-only cheby, "
+                                   "scaled by the copy kernel (profiles/hbm_traffic_256.md)") if traffic else None}
 
     # CPU baseline: the reference's own x86 solve path (compiled from its
     # sources into oracle/_ref) or, where that is absent, the oracle restatement.

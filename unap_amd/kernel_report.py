@@ -79,7 +79,8 @@ def report(trace_csv, bench_json):
     if not starts:
         raise SystemExit("no kStateInit dispatch in the trace")
     seg = rows[starts[-1]:]
-    info = json.load(open(bench_json))
+    # bench.py prints one JSON line; take the last one in the file
+    info = json.loads([ln for ln in open(bench_json) if ln.startswith("{")][-1])
     lv = _levels(info["hierarchy"])
     iters = int(info["iterations"])
     t0, t1 = int(seg[0]["Start_Timestamp"]), int(seg[-1]["End_Timestamp"])
@@ -108,7 +109,40 @@ def report(trace_csv, bench_json):
     out.append("")
     out.append(f"all kernels: {tot_b / 1e9:.1f} GB algorithmic in {busy / 1e6:.1f} ms busy = "
                f"{tot_b / (busy * 1e-9) / 1e9:.0f} GB/s")
+    out.append("")
+    out += per_level(seg, lv)
     return "\n".join(out)
+
+
+def _by_position(seg, pattern, order):
+    """Mean duration (ns) per level of the launches whose name contains
+    `pattern`, given the level of each launch within one V-cycle."""
+    ds = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in seg if pattern in r["Kernel_Name"]]
+    acc = collections.defaultdict(list)
+    for i, d in enumerate(ds):
+        acc[order[i % len(order)]].append(d)
+    return {lvl: sum(v) / len(v) for lvl, v in acc.items()}
+
+
+def per_level(seg, lv):
+    """Per-level means from the fixed launch order of a V-cycle: Chebyshev
+    sweeps go coarse to fine (4 a level, the coarsest level unsmoothed),
+    restrictions fine to coarse, prolongations coarse to fine."""
+    L = len(lv) - 1
+    cheb = _by_position(seg, "BCheby<float>", [lvl for lvl in range(L - 1, 0, -1) for _ in range(4)] + [0] * 4)
+    rst = _by_position(seg, "BRestrict<float, float>", list(range(L)))
+    pro = _by_position(seg, "BProlong<float>", list(range(L - 1, -1, -1)))
+    out = ["| level | cells | entries/row | Chebyshev us | Chebyshev GB/s | restrict to next us | "
+           "prolong from next us |",
+           "|---|---|---|---|---|---|---|"]
+    for lvl in range(L + 1):
+        n, f = lv[lvl]
+        c = cheb.get(lvl)
+        cs = f"{c / 1e3:.1f} | {(28 * n + 16 * f) / c:.0f}" if c else "- | -"
+        r = f"{rst[lvl] / 1e3:.1f}" if lvl in rst else "-"
+        p = f"{pro[lvl] / 1e3:.1f}" if lvl in pro else "-"
+        out.append(f"| {lvl} | {n} | {2 * f / n:.2f} | {cs} | {r} | {p} |")
+    return out
 
 
 if __name__ == "__main__":
