@@ -256,7 +256,47 @@ def batch_bench(nsys, launches, with_cpu):
            "setup_s": {"generate": round(t_gen, 2), "create": round(t_create, 2)}}
     if with_cpu:
         out["cpu_baseline"] = batch_cpu_baseline(systems, solvers, p50)
+        out["cpu_baseline_all_cores"] = batch_cpu_all_cores(nsys)
     return out
+
+
+def _cpu_solve_systems(args):
+    """Worker: solve the given systems of the batch on the CPU path, one after
+    another; returns the seconds spent (preconditioner setup + solve)."""
+    nsys, idx = args
+    from oracle import oracle as O
+    from unap_amd import meshgen
+    path = O.reference() or O.oracle()
+    specs = meshgen.batch_specs(nsys=nsys)
+    t = 0.0
+    for i in idx:
+        s = meshgen.batch_system(specs[i])
+        kind = "PCG" if specs[i][3] == meshgen.POISSON_LOGUNIF else "PBiCGStab"
+        _, t_setup, t_solve, _ = O.timed_solve(path, path.matrix(s), kind, "DIC" if kind == "PCG" else "DILU",
+                                               s.b, 1000, 1, 1e-8)
+        t += t_setup + t_solve
+    return t
+
+
+def batch_cpu_all_cores(nsys):
+    """All nsys systems solved by the CPU path, one independent single-threaded
+    solve per worker process (the reference runs one system per MPI rank),
+    over min(16, cores) worker processes; systems/s from the wall clock."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    from oracle import oracle as O
+    kind = "reference" if O.reference() is not None else "port"
+    workers = max(1, min(16, os.cpu_count() or 1))
+    chunks = [(nsys, list(range(w, nsys, workers))) for w in range(workers)]
+    t0 = time.perf_counter()
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        busy = list(ex.map(_cpu_solve_systems, chunks))
+    wall = time.perf_counter() - t0
+    log(f"batch cpu all cores ({kind}): {nsys} systems on {workers} processes in {wall:.1f}s wall")
+    return {"value": round(nsys / wall, 2), "unit": "systems/s", "cores": workers, "kind": kind,
+            "sample": f"all {nsys} systems, one FP64 solve per system (preconditioner setup + solve), round-robin "
+                      f"over {workers} worker processes; wall clock {wall:.1f}s including process start, "
+                      f"per-process busy {min(busy):.1f}-{max(busy):.1f}s"}
 
 
 def batch_cpu_baseline(systems, solvers, gpu_ms, stride=16):

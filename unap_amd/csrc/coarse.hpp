@@ -92,15 +92,75 @@ __device__ __forceinline__ void wgReduce(const CoarseArgs& a, int n, F term, dou
     wgSum<NS>(v, red);
 }
 
-__device__ __forceinline__ void wgPrecond(const CoarseArgs& a, double* w, const double* r) {
+// The coarsest matrix read in place (SELL layout in global memory).
+struct CoarseGlobal {
+    const CoarseArgs& a;
+    __device__ double amul(int i, const double* x) const { return rowAmul<double, double>(a.A, i, x); }
+    __device__ void fwd(int i, double* w, const double* r) const { fwdRow<double, double>(a.AP, i, w, r, a.rD); }
+    __device__ void bwd(int i, double* w) const { bwdRow<double, double>(a.AP, i, w, a.rD); }
+    __device__ int fs(int k) const { return a.fwdStart[k]; }
+    __device__ int frow(int q) const { return a.fwdRows ? a.fwdRows[q] : q; }
+    __device__ int bs(int k) const { return a.bwdStart[k]; }
+    __device__ int brow(int q) const { return a.bwdRows[q]; }
+};
+
+// The coarsest matrix in CSR form, built at setup: entries of a row in the
+// SELL order (ascending face id), mid = number of neighbour-side entries.
+struct CoarseCsr {
+    const int* start;
+    const uint16_t* col;
+    const uint16_t* mid;
+    const double* val;
+    int nnz;
+};
+
+// The same matrix, rD and level schedule copied into LDS at kernel start:
+// every row access of the sweeps and products is then an LDS access instead
+// of an L2 round trip.  The arithmetic is that of rowAmul / fwdRow / bwdRow.
+struct CoarseLocal {
+    const double* val;
+    const double* diag;
+    const double* rD;
+    const int* start;
+    const uint16_t* col;
+    const uint16_t* mid;
+    const int* fs_;
+    const int* frows;
+    const int* bs_;
+    const int* brows;
+    __device__ double amul(int i, const double* x) const {
+        double acc = 0.0 + diag[i] * x[i];
+        for (int k = start[i]; k < start[i + 1]; ++k) acc = acc + val[k] * x[col[k]];
+        return acc;
+    }
+    __device__ void fwd(int i, double* w, const double* r) const {
+        const double rd = rD[i];
+        double acc = rd * r[i];
+        const int e = start[i] + mid[i];
+        for (int k = start[i]; k < e; ++k) acc = acc - (rd * val[k]) * w[col[k]];
+        w[i] = acc;
+    }
+    __device__ void bwd(int i, double* w) const {
+        const double rd = rD[i];
+        double acc = w[i];
+        const int b = start[i] + mid[i];
+        for (int k = start[i + 1] - 1; k >= b; --k) acc = acc - (rd * val[k]) * w[col[k]];
+        w[i] = acc;
+    }
+    __device__ int fs(int k) const { return fs_[k]; }
+    __device__ int frow(int q) const { return frows ? frows[q] : q; }
+    __device__ int bs(int k) const { return bs_[k]; }
+    __device__ int brow(int q) const { return brows[q]; }
+};
+
+template <class M>
+__device__ __forceinline__ void wgPrecond(const CoarseArgs& a, const M& m, double* w, const double* r) {
     for (int k = 0; k < a.nFwd; ++k) {
-        for (int q = a.fwdStart[k] + threadIdx.x; q < a.fwdStart[k + 1]; q += blockDim.x)
-            fwdRow<double, double>(a.AP, a.fwdRows ? a.fwdRows[q] : q, w, r, a.rD);
+        for (int q = m.fs(k) + threadIdx.x; q < m.fs(k + 1); q += blockDim.x) m.fwd(m.frow(q), w, r);
         __syncthreads();
     }
     for (int k = 0; k < a.nBwd; ++k) {
-        for (int q = a.bwdStart[k] + threadIdx.x; q < a.bwdStart[k + 1]; q += blockDim.x)
-            bwdRow<double, double>(a.AP, a.bwdRows[q], w, a.rD);
+        for (int q = m.bs(k) + threadIdx.x; q < m.bs(k + 1); q += blockDim.x) m.bwd(m.brow(q), w);
         __syncthreads();
     }
 }
@@ -109,16 +169,11 @@ __device__ __forceinline__ bool wgConv(double fin, double init, double tol, doub
     return (fin < tol || (relTol > kSmallD && fin <= relTol * init)) && it >= 0;
 }
 
-// PCG + DIC (PCG.cpp:35-183) on the coarsest level.
-static __global__ void __launch_bounds__(1024) kCoarsePCG(CoarseArgs a, Skip sk) {
-    if (sk.on()) return;
-    extern __shared__ double lds[];
+// PCG + DIC (PCG.cpp:35-183) on the coarsest level; x, r, w, p in LDS.
+template <class M>
+__device__ __forceinline__ void coarsePCG(const CoarseArgs& a, const M& m, double* x, double* r, double* w,
+                                          double* p, double* red) {
     const int n = a.A.n;
-    double* x = lds;
-    double* r = x + n;
-    double* w = r + n;
-    double* p = w + n;
-    double* red = p + n;
     const int t = threadIdx.x, nt = blockDim.x;
     double s1[1];
     // x = 0; A x = 0 exactly, so r = b - A x = b
@@ -134,14 +189,14 @@ static __global__ void __launch_bounds__(1024) kCoarsePCG(CoarseArgs a, Skip sk)
     long long nIter = 0;
     for (;;) {
         rhoOld = rho;
-        wgPrecond(a, w, r);
+        wgPrecond(a, m, w, r);
         wgReduce<1>(a, n, [&](int i, double(&q)[1]) { q[0] = w[i] * r[i]; }, s1, red);
         rho = s1[0];
         if (!(fabs(rho) > kSmallD)) break;
         const double beta = rho / rhoOld;
         for (int i = t; i < n; i += nt) p[i] = (nIter == 0) ? w[i] : w[i] + beta * p[i];
         __syncthreads();
-        for (int i = t; i < n; i += nt) w[i] = rowAmul<double, double>(a.A, i, p);
+        for (int i = t; i < n; i += nt) w[i] = m.amul(i, p);
         __syncthreads();
         wgReduce<1>(a, n, [&](int i, double(&q)[1]) { q[0] = w[i] * p[i]; }, s1, red);
         const double alpha = rho / s1[0];
@@ -157,6 +212,61 @@ static __global__ void __launch_bounds__(1024) kCoarsePCG(CoarseArgs a, Skip sk)
         if (wgConv(fin, init, a.tol, a.relTol, nIter)) break;
     }
     for (int i = t; i < n; i += nt) a.corr[i] = x[i];
+}
+
+static __global__ void __launch_bounds__(1024) kCoarsePCG(CoarseArgs a, Skip sk) {
+    if (sk.on()) return;
+    extern __shared__ double lds[];
+    const int n = a.A.n;
+    coarsePCG(a, CoarseGlobal{a}, lds, lds + n, lds + 2 * n, lds + 3 * n, lds + 4 * n);
+}
+
+// LDS bytes of kCoarsePCGLocal for a level of n rows, nnz entries, nf / nb
+// forward / backward levels, with or without a forward row list.
+inline size_t coarseLocalBytes(int n, int nnz, int nf, int nb, bool fwdRows) {
+    const size_t d = sizeof(double) * ((size_t)nnz + 6 * (size_t)n + 64);
+    const size_t i = sizeof(int) * ((size_t)n + 1 + (fwdRows ? n : 0) + n + nf + 1 + nb + 1);
+    const size_t h = sizeof(uint16_t) * ((size_t)nnz + n);
+    return d + i + h + 16;
+}
+
+static __global__ void __launch_bounds__(256) kCoarsePCGLocal(CoarseArgs a, CoarseCsr c, Skip sk) {
+    if (sk.on()) return;
+    extern __shared__ double lds[];
+    const int n = a.A.n, nnz = c.nnz, t = threadIdx.x, nt = blockDim.x;
+    double* val = lds;
+    double* diag = val + nnz;
+    double* rD = diag + n;
+    double* x = rD + n;
+    double* r = x + n;
+    double* w = r + n;
+    double* p = w + n;
+    double* red = p + n;
+    int* start = reinterpret_cast<int*>(red + 64);
+    int* frows = start + n + 1;
+    int* brows = frows + (a.fwdRows ? n : 0);
+    int* fs = brows + n;
+    int* bs = fs + a.nFwd + 1;
+    uint16_t* col = reinterpret_cast<uint16_t*>(bs + a.nBwd + 1);
+    uint16_t* mid = col + nnz;
+    for (int k = t; k < nnz; k += nt) {
+        val[k] = c.val[k];
+        col[k] = c.col[k];
+    }
+    for (int i = t; i < n; i += nt) {
+        diag[i] = a.A.diag[i];
+        rD[i] = a.rD[i];
+        start[i] = c.start[i];
+        mid[i] = c.mid[i];
+        brows[i] = a.bwdRows[i];
+        if (a.fwdRows) frows[i] = a.fwdRows[i];
+    }
+    if (t == 0) start[n] = c.start[n];
+    for (int k = t; k <= a.nFwd; k += nt) fs[k] = a.fwdStart[k];
+    for (int k = t; k <= a.nBwd; k += nt) bs[k] = a.bwdStart[k];
+    __syncthreads();
+    const CoarseLocal m{val, diag, rD, start, col, mid, fs, a.fwdRows ? frows : nullptr, bs, brows};
+    coarsePCG(a, m, x, r, w, p, red);
 }
 
 // PBiCGStab + DILU (PBiCGStab.cpp:45-459) on the coarsest level.
@@ -197,7 +307,7 @@ static __global__ void __launch_bounds__(1024) kCoarseBiCG(CoarseArgs a, Skip sk
         for (int i = t; i < n; i += nt)
             p[i] = (nIter == 0) ? r[i] : r[i] + beta * (p[i] - omega * Ay[i]);
         __syncthreads();
-        wgPrecond(a, y, p);
+        wgPrecond(a, CoarseGlobal{a}, y, p);
         for (int i = t; i < n; i += nt) Ay[i] = rowAmul<double, double>(a.A, i, y);
         __syncthreads();
         wgReduce<1>(a, n, [&](int i, double(&q)[1]) { q[0] = r0[i] * Ay[i]; }, s1, red);
@@ -211,7 +321,7 @@ static __global__ void __launch_bounds__(1024) kCoarseBiCG(CoarseArgs a, Skip sk
             __syncthreads();
             break;
         }
-        wgPrecond(a, z, s);
+        wgPrecond(a, CoarseGlobal{a}, z, s);
         for (int i = t; i < n; i += nt) tt[i] = rowAmul<double, double>(a.A, i, z);
         __syncthreads();
         wgReduce<2>(a, n, [&](int i, double(&q)[2]) {

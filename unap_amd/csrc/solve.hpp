@@ -234,6 +234,12 @@ struct Hierarchy {
     std::unique_ptr<DevMatrix<double>> coarsest64;
     DBuf<double> rDc;
     size_t coarseLds = 0;
+    // LDS-resident copy of a symmetric coarsest level (kCoarsePCGLocal)
+    bool coarseLocal = false;
+    size_t coarseLocalLds = 0;
+    DBuf<int> csrStart;
+    DBuf<uint16_t> csrCol, csrMid;
+    DBuf<double> csrVal;
     // coarsest (FP32 path): dense inverse in internal order
     DBuf<double> Ainv;
 

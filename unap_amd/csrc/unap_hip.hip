@@ -168,6 +168,46 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
         else
             UH_CHECK(hipFuncSetAttribute((const void*)kCoarseBiCG,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)coarseLds));
+        // Symmetric coarsest level: a CSR copy that kCoarsePCGLocal keeps in
+        // LDS, when it fits the device's per-workgroup limit.
+        coarseLocal = false;
+        if (CA.symm && CL.n <= 65535) {
+            const int n = CL.n;
+            std::vector<int> st(n + 1, 0);
+            for (int i = 0; i < n; ++i) st[i + 1] = st[i] + (int)(CL.rowInfo[i] & 0xffffu);
+            const int nnz = st[n];
+            const size_t bytes = coarseLocalBytes(n, nnz, (int)CL.fwdStart.size() - 1, (int)CL.bwdStart.size() - 1,
+                                                  !CL.fwdRows.empty());
+            int dev = 0, optin = 0;
+            UH_CHECK(hipGetDevice(&dev));
+            if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) optin = 0;
+            if (bytes <= (size_t)optin &&
+                hipFuncSetAttribute((const void*)kCoarsePCGLocal, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)bytes) == hipSuccess) {
+                std::vector<double> sv;
+                fillSellValues<double>(CL, CA, false, sv);
+                std::vector<double> val(nnz);
+                std::vector<uint16_t> col(nnz), mid(n);
+                for (int i = 0; i < n; ++i) {
+                    const long long base = CL.sliceStart[i >> kSliceShift] + (i & (kSliceRows - 1));
+                    for (int k = 0; k < st[i + 1] - st[i]; ++k) {
+                        const long long q = base + ((long long)k << kSliceShift);
+                        val[st[i] + k] = sv[q];
+                        col[st[i] + k] = (uint16_t)CL.col[q];
+                    }
+                    mid[i] = (uint16_t)(CL.rowInfo[i] >> 16);
+                }
+                csrStart.upload(st, c.stream);
+                csrVal.upload(val, c.stream);
+                csrCol.upload(col, c.stream);
+                csrMid.upload(mid, c.stream);
+                c.sync();
+                coarseLocalLds = bytes;
+                coarseLocal = true;
+            } else {
+                (void)hipGetLastError();  // a refused attribute leaves no sticky error
+            }
+        }
     } else {
         // mixed path: dense FP64 inverse of the coarsest Galerkin matrix
         const int nc = CL.n;
@@ -258,7 +298,10 @@ void Hierarchy<T>::coarsestSolve(Skip sk) {
     a.maxIter = 100;
     a.iperm = cm->iperm.get();
     a.serial = c.serialReductions;
-    if (CA.symm)
+    if (CA.symm && coarseLocal)
+        kCoarsePCGLocal<<<1, 256, coarseLocalLds, c.stream>>>(
+            a, CoarseCsr{csrStart.get(), csrCol.get(), csrMid.get(), csrVal.get(), (int)csrVal.n}, sk);
+    else if (CA.symm)
         kCoarsePCG<<<1, 1024, coarseLds, c.stream>>>(a, sk);
     else
         kCoarseBiCG<<<1, 1024, coarseLds, c.stream>>>(a, sk);
