@@ -6,13 +6,17 @@ log-uniform in [1e-4, 1], solved by AMG-preconditioned CG to relative
 residual 1e-8 from x = 0.
 
   * mixed (the headline `value`): flexible CG in FP64 (true residual,
-    dot products and norms in FP64) around one V-cycle of the pairwise
-    agglomeration AMG in FP32 (hierarchy, Chebyshev(4) smoothers on every
-    level, restriction and prolongation in FP32; coarsest level solved in
-    FP64 by a dense inverse);
+    dot products and norms in FP64) around one V-cycle of the agglomeration
+    AMG in FP32 (every second level of the reference's pairwise hierarchy,
+    Chebyshev(4) smoothers on every level with estimates taken at setup,
+    restriction and prolongation in FP32; coarsest level solved in FP64 by a
+    dense inverse);
   * fp64 drop-in: the reference's algorithm in FP64 on the GPU (PCG around
-    MGSolver V-cycles with Chebyshev smoothers, nPost = nFinest = 4 with the
-    reference's nPost + level schedule, coarsest level by PCG+DIC to 1e-2).
+    MGSolver V-cycles over the full pairwise hierarchy with Chebyshev
+    smoothers, nPost = nFinest = 4 with the reference's nPost + level
+    schedule, coarsest level by PCG+DIC to 1e-2);
+  * fp64 same algorithm: the mixed path's algorithm with the hierarchy in
+    FP64, so that the mixed/FP64 ratio of precision alone is reported too.
 
 Inputs are resident in HBM before the timed region; each step is one full
 solve.  Prints one JSON line on stdout; progress goes to stderr.
@@ -43,6 +47,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--n-coarsest", type=int, default=500)
+    ap.add_argument("--pair-steps", type=int, default=2, help="pairwise steps per level of the mixed hierarchy")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-size", type=int, default=None, help="edge length of the CPU sample (default: --size)")
@@ -89,18 +94,24 @@ def main():
 
     common = dict(smoother=2, n_cells_coarsest=args.n_coarsest, n_vcycles=1, n_post_sweeps=4, n_finest_sweeps=4)
     t0 = time.time()
-    P32 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, level_sweep_growth=0, **common))
+    P32 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, level_sweep_growth=0, pair_steps=args.pair_steps,
+                                                   **common))
     t_setup32 = time.time() - t0
     log(f"mixed hierarchy setup {t_setup32:.1f}s (device layout, agglomeration, coarse layouts, estimates)")
     t0 = time.time()
     P64 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=64, level_sweep_growth=1, **common))
     t_setup64 = time.time() - t0
     log(f"fp64 hierarchy setup {t_setup64:.1f}s")
+    P64f = None
+    if not args.mixed_only:
+        P64f = hip.Precond(A, "AMG", amg=hip.amg_params(precision=64, level_sweep_growth=0, pair_steps=args.pair_steps,
+                                                        cycle=hip.CYCLE_FIXED, **common))
 
     b = hip.DeviceArray.from_numpy(s.b)
     x = hip.DeviceArray(s.n)
     S32 = hip.Solver("FCG", P32, rel_tol=1e-8, max_iter=1000)
     S64 = hip.Solver("PCG", P64, rel_tol=1e-8, max_iter=1000)
+    S64f = hip.Solver("FCG", P64f, rel_tol=1e-8, max_iter=1000) if P64f is not None else None
 
     def solve(S):
         x.zero()
@@ -132,6 +143,14 @@ def main():
         log(f"fp64 step {k}: {dt * 1e3:.1f} ms, {p64['nIterations']} iterations, converged={p64['converged']}")
     x64 = x.numpy()
     t64 = float(np.median(times64))
+    times64f = []
+    for k in range(max(1, args.fp64_steps)):
+        dt, p64f = solve(S64f)
+        times64f.append(dt)
+        log(f"fp64 same-algorithm step {k}: {dt * 1e3:.1f} ms, {p64f['nIterations']} iterations, "
+            f"converged={p64f['converged']}")
+    x64f = x.numpy()
+    t64f = float(np.median(times64f))
 
     # accuracy: against an FP64 solve to 1e-12 (flexible CG around the FP64
     # hierarchy: the reference's V-cycle is not a fixed linear operator, so
@@ -141,6 +160,7 @@ def main():
     xs = x.numpy()
     err32 = float(np.linalg.norm(x32 - xs) / np.linalg.norm(xs))
     err64 = float(np.linalg.norm(x64 - xs) / np.linalg.norm(xs))
+    err64f = float(np.linalg.norm(x64f - xs) / np.linalg.norm(xs))
     r32 = s.b - A.spmv(x32)
     true_res32 = float(np.linalg.norm(r32) / np.linalg.norm(s.b))
     log(f"errors vs FP64@1e-12 ({pref['nIterations']} it): mixed {err32:.2e}, fp64@1e-8 {err64:.2e}; "
@@ -173,7 +193,7 @@ def main():
 
     batch = None
     if not args.no_batch:
-        del P32, P64, S32, S64, Sref, A, x, b
+        del P32, P64, P64f, S32, S64, S64f, Sref, A, x, b
         batch = batch_bench(args.batch_systems, args.batch_launches, not args.no_cpu)
 
     t32 = float(np.median(times32))
@@ -182,15 +202,21 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(t32 * 1e3, 3), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"C4: {N}^3 permuted log-uniform(1e-4..1) Poisson LDU, AMG-PCG to rel.res 1e-8",
-                   "n_cells": s.n, "n_faces": s.nfaces, "mixed": "FCG(FP64) + FP32 AMG V(0,4) Chebyshev(4), "
-                   "coarsest dense FP64", "fp64": "PCG + FP64 AMG V(0,4+level) Chebyshev(4), coarsest PCG+DIC 1e-2",
+                   "n_cells": s.n, "n_faces": s.nfaces,
+                   "mixed": f"FCG(FP64) + FP32 AMG V(0,4) Chebyshev(4), {args.pair_steps} pairwise steps a level, "
+                            "coarsest dense FP64",
+                   "fp64": "PCG + FP64 AMG V(0,4+level) Chebyshev(4), reference pairwise hierarchy, coarsest PCG+DIC 1e-2",
+                   "fp64_same_algorithm": "the mixed algorithm with the hierarchy in FP64",
                    "n_cells_coarsest": args.n_coarsest},
         "hierarchy_levels": levels32,
         "mixed": {"s": round(t32, 5), "device_ms": round(ms_dev32, 3), "iterations": p32["nIterations"],
                   "converged": p32["converged"], "err_vs_fp64_1e-12": err32, "true_rel_residual": true_res32},
         "fp64_drop_in": {"s": round(t64, 5), "iterations": p64["nIterations"], "converged": p64["converged"],
                          "err_vs_fp64_1e-12": err64},
+        "fp64_same_algorithm": {"s": round(t64f, 5), "iterations": p64f["nIterations"],
+                                "converged": p64f["converged"], "err_vs_fp64_1e-12": err64f},
         "speedup_mixed_vs_fp64": round(t64 / t32, 3),
+        "speedup_mixed_vs_fp64_same_algorithm": round(t64f / t32, 3),
         "setup_s": {"generate": round(t_gen, 2), "ldu_create": round(t_create, 2),
                     "mixed_hierarchy": round(t_setup32, 2), "fp64_hierarchy": round(t_setup64, 2)},
         "device": dev_name, "roofline": roofline, "kernels": kernels, "cpu_baseline": cpu,

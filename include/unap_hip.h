@@ -100,7 +100,8 @@ int unap_hip_ldu_spmv(unap_hip_ldu A, const double* psi, double* Apsi, int where
    PCG+DIC / PBiCGStab+DILU to relTol 1e-2, solveCoarsestLevel.cpp:10-55).
    precision 32: FP32 hierarchy, smoothers and transfers, Chebyshev estimates
    taken once at setup, coarsest level solved directly in FP64 (a dense
-   Cholesky inverse); use it under the flexible solver UNAP_HIP_SOLVER_FCG. */
+   inverse by Gauss-Jordan elimination at setup); use it under the flexible
+   solver UNAP_HIP_SOLVER_FCG. */
 typedef struct {
     int64_t smoother;               /* UNAP_HIP_SMOOTHER_* (default GS) */
     int64_t n_cells_coarsest;       /* SET_nCellsInCoarsestLevel (default 50) */
@@ -112,7 +113,18 @@ typedef struct {
     int64_t precision;              /* 64 or 32 */
     int64_t level_sweep_growth;     /* 1: coarse level l smooths nPost + l times (Vcycle.cpp:250,
                                        default); 0: every coarse level smooths nPost times */
+    int64_t pair_steps;             /* pairwise agglomeration steps per hierarchy level: 1 (the
+                                       reference, default) or 2 (keep every second level of the
+                                       reference hierarchy, restriction maps composed) */
+    int64_t cycle;                  /* UNAP_HIP_CYCLE_*: the reference V-cycle (Chebyshev
+                                       estimates on each solve's first call, coarsest level by
+                                       PCG/PBiCGStab to 1e-2) or a fixed one (estimates once at
+                                       setup, coarsest level by a dense FP64 inverse); default
+                                       UNAP_HIP_CYCLE_AUTO: reference in FP64, fixed in FP32 */
 } unap_hip_amg_params;
+#define UNAP_HIP_CYCLE_AUTO (-1)
+#define UNAP_HIP_CYCLE_REFERENCE 0
+#define UNAP_HIP_CYCLE_FIXED 1
 
 int unap_hip_amg_params_default(unap_hip_amg_params* p);
 int unap_hip_precond_create(unap_hip_ldu A, int type, const unap_hip_amg_params* amg,

@@ -231,11 +231,16 @@ def test_fcg_fp64_parity(oracle, order, pre, mg):
     _check_solve(S.solve(A, s.b), ref, ("FCG", pre), order)
 
 
-def test_mixed_fcg_accuracy(oracle):
+@pytest.mark.parametrize("precision,pair_steps", [(32, 1), (32, 2), (64, 2)])
+def test_mixed_fcg_accuracy(oracle, precision, pair_steps):
+    # FP32 hierarchy (and the same fixed-cycle algorithm in FP64) under FCG:
+    # the true residual meets the tolerance and the error against the FP64
+    # oracle solution is that of an FP64 solve at the same tolerance
     s = _sys(4, 20)
     A = hip.LduMatrix.from_system(s)
     P = hip.Precond(A, "AMG", amg=hip.amg_params(smoother=2, n_cells_coarsest=50, n_vcycles=1,
-                                                   n_post_sweeps=4, n_finest_sweeps=4, precision=32))
+                                                   n_post_sweeps=4, n_finest_sweeps=4, precision=precision,
+                                                   pair_steps=pair_steps, cycle=hip.CYCLE_FIXED))
     S = hip.Solver("FCG", P, rel_tol=1e-8, max_iter=300)
     x, p = S.solve(A, s.b)
     assert p["converged"]
@@ -247,3 +252,12 @@ def test_mixed_fcg_accuracy(oracle):
     e_mixed = np.linalg.norm(x - xo) / np.linalg.norm(xo)
     e_fp64 = np.linalg.norm(x8 - xo) / np.linalg.norm(xo)
     assert e_mixed <= 10 * e_fp64 + 1e-12, (e_mixed, e_fp64)
+    lv = P.amg_levels()
+    assert lv[0] == (s.n, s.nfaces) and all(a[0] > b[0] for a, b in zip(lv, lv[1:]))
+
+
+def test_fp32_hierarchy_needs_fixed_cycle():
+    s = _sys(4, 10)
+    A = hip.LduMatrix.from_system(s)
+    with pytest.raises(hip.UnapHipError):
+        hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, cycle=hip.CYCLE_REFERENCE))

@@ -206,6 +206,8 @@ struct AmgParams {
     int smoother = 1, nPre = 0, nPost = 2, nFinest = 2, nVcycles = 2, precision = 64;
     int growth = 1;  // coarse level l smooths nPost + l (pre: nPre + l) times
     int maxLevels = 50, nCoarsest = 50;
+    int pairSteps = 1;  // agglomeration steps per hierarchy level
+    bool fixedCycle = false;  // estimates at setup, dense FP64 coarsest solve
 };
 
 // The V-cycle of MGSolver (Vcycle.cpp:27-332) in precision T on a resident
@@ -243,12 +245,17 @@ struct Hierarchy {
     // coarsest (FP32 path): dense inverse in internal order
     DBuf<double> Ainv;
 
-    int nl(int l) const { return l < 0 ? fine->n : ag->levels[l].nCoarse; }
+    std::vector<int> sel;  // hierarchy coarse level l = agglomeration level sel[l]
+    const CoarseLevel& lev(int l) const { return ag->levels[sel[l]]; }
+    int nl(int l) const { return l < 0 ? fine->n : lev(l).nCoarse; }
     DevMatrix<T>& mat(int l) { return l < 0 ? *fine : *mats[l]; }
 
     void build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgParams& p);
+    // The reference re-arms the Chebyshev estimate on every solve; a fixed
+    // cycle keeps the estimates taken at setup.
     void initSmoothers() {
-        for (auto& f : firstCall) f = 1;
+        if (!prm.fixedCycle)
+            for (auto& f : firstCall) f = 1;
     }
     void smooth(int si, T* x, const T* b, int nSweeps, Skip sk);
     void coarsestSolve(Skip sk);

@@ -104,11 +104,18 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     ag = agg;
     prm = p;
     const std::vector<Layout>& lays = ag->layoutsFor(order);
-    L = (int)ag->levels.size();
+    // hierarchy levels: every pairSteps-th agglomeration level, ending at the
+    // coarsest one
+    sel.clear();
+    for (int k = (int)ag->levels.size() - 1; k >= 0; k -= p.pairSteps) sel.push_back(k);
+    std::reverse(sel.begin(), sel.end());
+    L = (int)sel.size();
     if (L == 0)
         throw Error(-4, "agglomeration produced no coarse level (matrix smaller than the coarsest-level size)");
     scale = A.host.symm;  // scaleCorrection_(A.symm()), MG.cpp:14
     const bool fp32 = sizeof(T) == 4;
+    const bool fixed = prm.fixedCycle;
+    if (fp32 && !fixed) throw Error(UNAP_HIP_ERR_ARG, "the FP32 hierarchy needs the fixed cycle");
     mats.clear();
     cs.clear();
     ch.clear();
@@ -117,9 +124,15 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     mats.resize(L);
     work.resize(L);
     for (int l = 0; l < L; ++l) {
-        const Layout& fl = (l == 0) ? A.layout(order) : lays[l - 1];
-        const Layout& cl = lays[l];
-        Transfer tr = buildTransfer(fl, cl, ag->levels[l].restrictMap);
+        const Layout& fl = (l == 0) ? A.layout(order) : lays[sel[l - 1]];
+        const Layout& cl = lays[sel[l]];
+        // restriction map from the previous hierarchy level: the composition
+        // of the agglomeration maps in between
+        const int k0 = (l == 0) ? 0 : sel[l - 1] + 1;
+        std::vector<int> rm = ag->levels[k0].restrictMap;
+        for (int k = k0 + 1; k <= sel[l]; ++k)
+            for (int& v : rm) v = ag->levels[k].restrictMap[v];
+        Transfer tr = buildTransfer(fl, cl, rm);
         cs.emplace_back();
         ch.emplace_back();
         f2c.emplace_back();
@@ -127,11 +140,11 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
         ch.back().upload(tr.child, c.stream);
         f2c.back().upload(tr.fineToCoarse, c.stream);
         c.sync();
-        if (!(fp32 && l == L - 1)) {
+        if (!(fixed && l == L - 1)) {
             mats[l].reset(new DevMatrix<T>);
-            mats[l]->build(ag->levels[l].A, cl, c.stream);
+            mats[l]->build(lev(l).A, cl, c.stream);
         }
-        work[l].ensure(ag->levels[l].nCoarse);
+        work[l].ensure(lev(l).nCoarse);
     }
     fw.ensure(fine->n);
     fres.alloc(fine->n);
@@ -139,15 +152,15 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     pre.clear();
     if (prm.nPre) {
         pre.resize(L);
-        for (int l = 0; l < L; ++l) pre[l].alloc(ag->levels[l].nCoarse);
+        for (int l = 0; l < L; ++l) pre[l].alloc(lev(l).nCoarse);
     }
     firstCall.assign(L, 1);
     est.alloc(L);
     mgst.alloc(1);
 
-    const HostLdu& CA = ag->levels[L - 1].A;
-    const Layout& CL = lays[L - 1];
-    if (!fp32) {
+    const HostLdu& CA = lev(L - 1).A;
+    const Layout& CL = lays[sel[L - 1]];
+    if (!fixed) {
         // reference path: PCG+DIC / PBiCGStab+DILU to relTol 1e-2 in one workgroup
         coarsest64.reset();
         DevMatrix<double>* cm = reinterpret_cast<DevMatrix<double>*>(mats[L - 1].get());
@@ -258,7 +271,7 @@ void Hierarchy<T>::smooth(int si, T* x, const T* b, int nSweeps, Skip sk) {
     T* v1 = (si == 0) ? W.a.get() : W.c.get();
     T* v2 = (si == 0) ? W.b.get() : W.d.get();
     T* v3 = (si == 0) ? W.c.get() : W.e.get();
-    if (firstCall[si] && sizeof(T) == 8) {
+    if (firstCall[si] && !prm.fixedCycle) {
         eigenEstimate<T>(M, x, b, 10, est.get() + si, v1, v2, v3, sk);
         firstCall[si] = 0;
         return;
@@ -272,15 +285,15 @@ void Hierarchy<T>::coarsestSolve(Skip sk) {
     const int l = L - 1;
     T* corr = work[l].a.get();
     const T* src = work[l].b.get();
-    const int nc = ag->levels[l].nCoarse;
-    if (sizeof(T) == 4) {
+    const int nc = lev(l).nCoarse;
+    if (prm.fixedCycle) {
         const int g = (nc + 3) / 4;
         kDenseSolve<T><<<g, 256, 0, c.stream>>>(Ainv.get(), src, corr, nc, sk);
         UH_LAUNCHED();
         return;
     }
     DevMatrix<double>* cm = reinterpret_cast<DevMatrix<double>*>(mats[l].get());
-    const HostLdu& CA = ag->levels[l].A;
+    const HostLdu& CA = lev(l).A;
     CoarseArgs a;
     a.A = cm->view();
     a.AP = CA.symm ? cm->viewU() : cm->view();
@@ -789,6 +802,8 @@ int unap_hip_amg_params_default(unap_hip_amg_params* p) {
         p->n_finest_sweeps = 2;
         p->precision = 64;
         p->level_sweep_growth = 1;
+        p->pair_steps = 1;
+        p->cycle = UNAP_HIP_CYCLE_AUTO;
     });
 }
 
@@ -804,7 +819,12 @@ static AmgParams toParams(const unap_hip_amg_params* p) {
     a.nFinest = (int)p->n_finest_sweeps;
     a.precision = (int)p->precision;
     a.growth = (int)p->level_sweep_growth;
+    a.pairSteps = (int)p->pair_steps;
     need(a.growth == 0 || a.growth == 1, "level_sweep_growth must be 0 or 1");
+    need(a.pairSteps >= 1 && a.pairSteps <= 3, "pair_steps must be 1, 2 or 3");
+    need(p->cycle >= -1 && p->cycle <= 1, "unknown cycle");
+    a.fixedCycle = (p->cycle == UNAP_HIP_CYCLE_AUTO) ? (a.precision == 32) : (p->cycle == UNAP_HIP_CYCLE_FIXED);
+    need(a.fixedCycle || a.precision == 64, "the FP32 hierarchy needs the fixed cycle");
     need(a.smoother == 1 || a.smoother == 2, "smoother must be GS (1) or Chebyshev (2)");
     need(a.precision == 64 || a.precision == 32, "precision must be 64 or 32");
     need(a.nVcycles >= 1 && a.nPre >= 0 && a.nPost >= 0 && a.nFinest >= 0, "bad sweep counts");
@@ -895,10 +915,11 @@ int unap_hip_precond_amg_levels(unap_hip_precond P, int64_t cap, int64_t* nLevel
         const PrecondObj& o = P->o;
         const AgglHost* ag = o.amg32 ? o.amg32->ag.get() : (o.amg64 ? o.amg64->ag.get() : nullptr);
         if (!ag) throw Error(UNAP_HIP_ERR_STATE, "not an AMG preconditioner");
-        const int64_t L = 1 + (int64_t)ag->levels.size();
+        const std::vector<int>& sel = o.amg32 ? o.amg32->sel : o.amg64->sel;
+        const int64_t L = 1 + (int64_t)sel.size();
         *nLevels = L;
         for (int64_t l = 0; l < std::min<int64_t>(L, cap); ++l) {
-            const HostLdu& H = (l == 0) ? o.A->host : ag->levels[l - 1].A;
+            const HostLdu& H = (l == 0) ? o.A->host : ag->levels[sel[l - 1]].A;
             if (nCells) nCells[l] = H.n;
             if (nFaces) nFaces[l] = H.nFaces();
         }

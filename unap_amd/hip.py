@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libunap_hip.so")
 
 HOST, DEVICE = 0, 1
+CYCLE_AUTO, CYCLE_REFERENCE, CYCLE_FIXED = -1, 0, 1
 PRECOND = {"none": 0, "diag": 1, "DIC": 2, "DILU": 3, "AMG": 4}
 SMOOTHER = {"GS": 1, "Cheby": 2}
 SOLVER = {"PCG": 1, "PBiCGStab": 2, "MG": 3, "FCG": 4}
@@ -30,7 +31,8 @@ _intp = ctypes.POINTER(ctypes.c_int)
 class AmgParams(ctypes.Structure):
     _fields_ = [("smoother", _i64), ("n_cells_coarsest", _i64), ("max_levels", _i64),
                 ("n_vcycles", _i64), ("n_pre_sweeps", _i64), ("n_post_sweeps", _i64),
-                ("n_finest_sweeps", _i64), ("precision", _i64), ("level_sweep_growth", _i64)]
+                ("n_finest_sweeps", _i64), ("precision", _i64), ("level_sweep_growth", _i64),
+                ("pair_steps", _i64), ("cycle", _i64)]
 
 
 class Perf(ctypes.Structure):
