@@ -117,10 +117,15 @@ void launchReduce(const Body& b, int n, const int* order, const Fin& fin, Skip s
     UH_LAUNCHED();
     int nb = L.grid;
     if (c.serialReductions) {
+        // reference order: one thread sums the terms serially
         kSerialSum<Body, NS><<<1, 64, 0, c.stream>>>(b, order, n, c.partials.get(), sk);
         UH_LAUNCHED();
         nb = 1;
     }
+    // A separate one-block finalize: folding it into the row kernel (the last
+    // block to finish sums the partials) needs a device-scope fence in every
+    // block, which waits for the block's stores and made the large reductions
+    // 60-90% slower.
     kFinalize<Fin, NS><<<1, kBlock, 0, c.stream>>>(fin, c.partials.get(), nb, sk);
     UH_LAUNCHED();
 }

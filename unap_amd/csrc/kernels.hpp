@@ -161,6 +161,21 @@ __global__ void __launch_bounds__(kBlock) kRowsReduce(Body body, int n, double* 
     }
 }
 
+// Sum partials[s * kMaxBlocks + 0 .. nblocks) per slot, thread-strided then
+// the block tree, and hand the totals to fin (thread 0).
+template <class Fin, int NS>
+__device__ __forceinline__ void finishPartials(const Fin& fin, const double* partials, int nblocks, double* sm) {
+    double acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double a = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) a += partials[s * kMaxBlocks + b];
+        acc[s] = a;
+    }
+    blockReduce<NS>(acc, sm);
+    if (threadIdx.x == 0) fin(acc);
+}
+
 // Row kernel without sums.
 template <class Body>
 __global__ void __launch_bounds__(kBlock) kRows(Body body, int n, Skip skip) {
@@ -196,15 +211,7 @@ __global__ void __launch_bounds__(kBlock) kFinalize(Fin fin, const double* __res
                                                     int nblocks, Skip skip) {
     if (skip.on()) return;
     __shared__ double sm[kMaxSums * 16];
-    double acc[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        double a = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) a += partials[s * kMaxBlocks + b];
-        acc[s] = a;
-    }
-    blockReduce<NS>(acc, sm);
-    if (threadIdx.x == 0) fin(acc);
+    finishPartials<Fin, NS>(fin, partials, nblocks, sm);
 }
 
 // ------------------------------------------------------ convergence helpers

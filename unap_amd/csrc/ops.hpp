@@ -126,11 +126,13 @@ struct BPcgXR {
     const double* p;
     const double* w;
     const KState* st;
+    float* r32 = nullptr;  // also store r in FP32 (the input of an FP32 preconditioner)
     __device__ void operator()(int i, double (&a)[1]) const {
         const double al = st->alpha;
         x[i] += al * p[i];
         const double ri = r[i] - al * w[i];
         r[i] = ri;
+        if (r32) r32[i] = (float)ri;
         a[0] += ri * ri;
     }
     __device__ void terms(int i, double (&t)[1]) const { t[0] = r[i] * r[i]; }
@@ -300,11 +302,12 @@ struct BPsiScaled {
     const T* d;
     const KState* st;
     int slot;
+    bool psiZero = false;  // psi is known to be zero: write fsf * c without reading it
     __device__ void operator()(int i) const {
         const T fsf = (T)st->sf[slot];
         const T c = fc[i] + (fres[i] - Apsi[i]) / d[i];
         fc[i] = c;
-        psi[i] = psi[i] + fsf * c;
+        psi[i] = psiZero ? fsf * c : psi[i] + fsf * c;
     }
 };
 template <typename T>

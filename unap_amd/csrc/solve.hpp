@@ -259,8 +259,9 @@ struct Hierarchy {
     }
     void smooth(int si, T* x, const T* b, int nSweeps, Skip sk);
     void coarsestSolve(Skip sk);
-    void vcycle(T* psi, const T* source, T* fRes, const double* fRes64, Skip sk);
-    void precondition(const double* r, T* w, Skip sk);
+    void vcycle(T* psi, const T* source, T* fRes, bool psiZero, Skip sk);
+    // w = M^-1 r.  r32: r already in FP32 (FP32 hierarchy), or null.
+    void precondition(const double* r, T* w, Skip sk, const T* r32 = nullptr);
 };
 
 // ------------------------------------------------------------ preconditioner

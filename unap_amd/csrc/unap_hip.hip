@@ -322,13 +322,10 @@ void Hierarchy<T>::coarsestSolve(Skip sk) {
 }
 
 template <typename T>
-void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, const double* castFrom, Skip sk) {
+void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, bool psiZero, Skip sk) {
     const int coarsest = L - 1;
     // restrict the finest residual (Vcycle.cpp:43-45)
-    if (castFrom)
-        launchRows(BRestrict<double, T>{cs[0].get(), ch[0].get(), castFrom, work[0].b.get(), fRes}, nl(0), sk);
-    else
-        launchRows(BRestrict<T, T>{cs[0].get(), ch[0].get(), fRes, work[0].b.get(), nullptr}, nl(0), sk);
+    launchRows(BRestrict<T, T>{cs[0].get(), ch[0].get(), fRes, work[0].b.get(), nullptr}, nl(0), sk);
     for (int l = 0; l < coarsest; ++l) {
         LevelWork<T>& W = work[l];
         const int n = nl(l);
@@ -368,7 +365,10 @@ void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, const double* castFr
     launchRows(BProlong<T>{f2c[0].get(), work[0].a.get(), fw.a.get()}, n, sk);
     if (scale) {
         launchReduce<2>(BSfAmul<T>{fine->view(), fw.a.get(), fRes, fw.b.get()}, n, fine->iperm.get(), FinSF{mgst.get(), 63}, sk);
-        launchRows(BPsiScaled<T>{psi, fw.a.get(), fRes, fw.b.get(), fine->diag.get(), mgst.get(), 63}, n, sk);
+        launchRows(BPsiScaled<T>{psi, fw.a.get(), fRes, fw.b.get(), fine->diag.get(), mgst.get(), 63, psiZero}, n,
+                   sk);
+    } else if (psiZero) {
+        launchRows(BCopy<T, T>{fw.a.get(), psi}, n, sk);
     } else {
         launchRows(BAddTo<T>{psi, fw.a.get()}, n, sk);
     }
@@ -378,20 +378,28 @@ void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, const double* castFr
 // precondition(w, r): w = 0; nVcycles V-cycles of MGSolver::solve(w, A, r)
 // with tol = relTol = 0 (the residual after the last cycle is never used).
 template <typename T>
-void Hierarchy<T>::precondition(const double* r, T* w, Skip sk) {
+void Hierarchy<T>::precondition(const double* r, T* w, Skip sk, const T* r32) {
     const int n = fine->n;
-    launchRows(BFill<T>{w, (T)0}, n, sk);
+    // The reference starts from w = 0 and adds the first correction; a fixed
+    // cycle writes the first correction directly (no fill).
+    const bool psiZero = prm.fixedCycle;
+    if (!psiZero) launchRows(BFill<T>{w, (T)0}, n, sk);
     const bool fp32 = sizeof(T) == 4;
     T* source = fp32 ? src32.get() : reinterpret_cast<T*>(const_cast<double*>(r));
-    // FP32 hierarchy: the FP64 residual is cast once, coalesced, and the
-    // restriction gathers FP32 values
-    if (fp32) launchRows(BCopy<double, T>{r, src32.get()}, n, sk);
+    if (fp32) {
+        // FP32 hierarchy: the residual in FP32, from the caller (stored by the
+        // Krylov update) or cast here once, coalesced
+        if (r32)
+            source = const_cast<T*>(r32);
+        else
+            launchRows(BCopy<double, T>{r, src32.get()}, n, sk);
+    }
     for (int cyc = 0; cyc < prm.nVcycles; ++cyc) {
         if (cyc == 0) {
-            vcycle(w, source, source, nullptr, sk);
+            vcycle(w, source, source, psiZero, sk);
         } else {
             launchRows(BResidNoNorm<T>{fine->view(), w, source, fw.b.get(), fres.get()}, n, sk);
-            vcycle(w, source, fres.get(), nullptr, sk);
+            vcycle(w, source, fres.get(), false, sk);
         }
     }
 }
@@ -432,7 +440,7 @@ void applyPrecond64(PrecondObj& P, const double* r, double* w, Skip sk) {
 struct SolveBufs {
     int n = -1;
     DBuf<double> x, b, v[8];
-    DBuf<float> z32[2];
+    DBuf<float> z32[2], r32;
     DBuf<double> hist;
     DBuf<KState> st;
     void ensure(int nn, long long histCap, bool f32) {
@@ -443,10 +451,12 @@ struct SolveBufs {
             n = nn;
             z32[0].release();
             z32[1].release();
+            r32.release();
         }
         if (f32 && !z32[0].p) {
             z32[0].alloc(nn);
             z32[1].alloc(nn);
+            r32.alloc(nn);
         }
         if ((long long)hist.n < histCap) hist.alloc(histCap);
         if (!st.p) st.alloc(1);
@@ -542,10 +552,13 @@ struct Driver {
         });
     }
 
+    // r32: an FP32 copy of r kept up to date by the residual update (the
+    // input of an FP32 preconditioner), or null.
     template <typename TZ>
-    void fcgWith(TZ* z0, TZ* z1, std::function<void(const double*, TZ*, Skip)> prec) {
+    void fcgWith(TZ* z0, TZ* z1, float* r32, std::function<void(const double*, TZ*, Skip)> prec) {
         double *x = B.x.get(), *b = B.b.get(), *r = B.v[0].get(), *q = B.v[1].get(), *p = B.v[2].get();
         launchReduce<1>(BResidual<double>{M.view(), x, b, r, nullptr}, n, M.iperm.get(), FinInitRes{st, 0});
+        if (r32) launchRows(BCopy<double, float>{r, r32}, n);
         const Skip sk{&st->done, nullptr};
         TZ* z = z0;
         TZ* zo = z1;
@@ -554,7 +567,7 @@ struct Driver {
             launchReduce<2>(BFcgRz<TZ>{r, z, zo, it == 0 ? 1 : 0}, n, M.iperm.get(), FinFcgRz{st}, sk);
             launchRows(BFcgP<TZ>{z, p, st}, n, sk);
             launchReduce<1>(BAmulDot{M.view(), p, q}, n, M.iperm.get(), FinPcgAlpha{st}, sk);
-            launchReduce<1>(BPcgXR{x, r, p, q, st}, n, M.iperm.get(), FinResNorm{st}, sk);
+            launchReduce<1>(BPcgXR{x, r, p, q, st, r32}, n, M.iperm.get(), FinResNorm{st}, sk);
             std::swap(z, zo);
         });
     }
@@ -562,10 +575,11 @@ struct Driver {
     void fcg() {
         if (S.P && S.P->amg32) {
             Hierarchy<float>* H = S.P->amg32.get();
-            fcgWith<float>(B.z32[0].get(), B.z32[1].get(),
-                           [H](const double* r, float* z, Skip sk) { H->precondition(r, z, sk); });
+            float* r32 = B.r32.get();
+            fcgWith<float>(B.z32[0].get(), B.z32[1].get(), r32,
+                           [H, r32](const double* r, float* z, Skip sk) { H->precondition(r, z, sk, r32); });
         } else {
-            fcgWith<double>(B.v[3].get(), B.v[4].get(),
+            fcgWith<double>(B.v[3].get(), B.v[4].get(), nullptr,
                             [this](const double* r, double* z, Skip sk) { precondInto(r, z, sk); });
         }
     }
@@ -576,7 +590,7 @@ struct Driver {
         launchReduce<1>(BResidual<double>{M.view(), x, b, fres, nullptr}, n, M.iperm.get(), FinInitRes{st, 1});
         const Skip sk{&st->done, nullptr};
         iterate([&](long long) {
-            H.vcycle(x, b, fres, nullptr, sk);
+            H.vcycle(x, b, fres, false, sk);
             launchReduce<1>(BResidual<double>{M.view(), x, b, fres, nullptr}, n, M.iperm.get(), FinResNorm{st}, sk);
         });
     }
