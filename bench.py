@@ -46,7 +46,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--n-coarsest", type=int, default=500)
+    ap.add_argument("--n-coarsest", type=int, default=500, help="coarsest-level threshold of the FP64 drop-in")
+    ap.add_argument("--mixed-n-coarsest", type=int, default=2000,
+                    help="coarsest-level threshold of the mixed algorithm (dense FP64 inverse)")
     ap.add_argument("--pair-steps", type=int, default=2, help="pairwise steps per level of the mixed hierarchy")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
@@ -92,20 +94,21 @@ def main():
     info = A.info()
     log(f"ldu_create {t_create:.1f}s", info)
 
-    common = dict(smoother=2, n_cells_coarsest=args.n_coarsest, n_vcycles=1, n_post_sweeps=4, n_finest_sweeps=4)
+    common = dict(smoother=2, n_vcycles=1, n_post_sweeps=4, n_finest_sweeps=4)
+    mixed = dict(level_sweep_growth=0, pair_steps=args.pair_steps, n_cells_coarsest=args.mixed_n_coarsest,
+                 cycle=hip.CYCLE_FIXED, **common)
     t0 = time.time()
-    P32 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, level_sweep_growth=0, pair_steps=args.pair_steps,
-                                                   **common))
+    P32 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, **mixed))
     t_setup32 = time.time() - t0
     log(f"mixed hierarchy setup {t_setup32:.1f}s (device layout, agglomeration, coarse layouts, estimates)")
     t0 = time.time()
-    P64 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=64, level_sweep_growth=1, **common))
+    P64 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=64, level_sweep_growth=1, n_cells_coarsest=args.n_coarsest,
+                                                   **common))
     t_setup64 = time.time() - t0
     log(f"fp64 hierarchy setup {t_setup64:.1f}s")
     P64f = None
     if not args.mixed_only:
-        P64f = hip.Precond(A, "AMG", amg=hip.amg_params(precision=64, level_sweep_growth=0, pair_steps=args.pair_steps,
-                                                        cycle=hip.CYCLE_FIXED, **common))
+        P64f = hip.Precond(A, "AMG", amg=hip.amg_params(precision=64, **mixed))
 
     b = hip.DeviceArray.from_numpy(s.b)
     x = hip.DeviceArray(s.n)
@@ -204,10 +207,10 @@ def main():
         "config": {"workload": f"C4: {N}^3 permuted log-uniform(1e-4..1) Poisson LDU, AMG-PCG to rel.res 1e-8",
                    "n_cells": s.n, "n_faces": s.nfaces,
                    "mixed": f"FCG(FP64) + FP32 AMG V(0,4) Chebyshev(4), {args.pair_steps} pairwise steps a level, "
-                            "coarsest dense FP64",
+                            f"coarsest threshold {args.mixed_n_coarsest}, dense FP64 inverse (GPU Gauss-Jordan)",
                    "fp64": "PCG + FP64 AMG V(0,4+level) Chebyshev(4), reference pairwise hierarchy, coarsest PCG+DIC 1e-2",
                    "fp64_same_algorithm": "the mixed algorithm with the hierarchy in FP64",
-                   "n_cells_coarsest": args.n_coarsest},
+                   "n_cells_coarsest_fp64": args.n_coarsest, "n_cells_coarsest_mixed": args.mixed_n_coarsest},
         "hierarchy_levels": levels32,
         "mixed": {"s": round(t32, 5), "device_ms": round(ms_dev32, 3), "iterations": p32["nIterations"],
                   "converged": p32["converged"], "err_vs_fp64_1e-12": err32, "true_rel_residual": true_res32},

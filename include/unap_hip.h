@@ -133,6 +133,11 @@ int unap_hip_precond_destroy(unap_hip_precond P);
 int unap_hip_precond_apply(unap_hip_precond P, const double* r, double* w, int where);
 /* rD() (unapMatrix.hpp:343): reciprocal (preconditioned) diagonal, host copy. */
 int unap_hip_precond_rd(unap_hip_precond P, double* rD);
+/* Inverse of a dense symmetric positive definite n x n matrix (row-major,
+   host arrays), by the blocked Gauss-Jordan inversion on FP64 MFMA that the
+   fixed V-cycle uses for its coarsest level.  No counterpart in the
+   reference (its coarsest level is solved iteratively, solveCoarsestLevel.cpp). */
+int unap_hip_dense_inverse_spd(int64_t n, const double* a, double* inv);
 /* Sizes of an AMG preconditioner's hierarchy: level 0 is the matrix itself,
    level l >= 1 the coarse level of agglomeration step l (the reference's
    coarseMatrixLevels(l - 1), lduAgglomeration.hpp:83-92).  Writes up to cap

@@ -256,6 +256,31 @@ def test_mixed_fcg_accuracy(oracle, precision, pair_steps):
     assert lv[0] == (s.n, s.nfaces) and all(a[0] > b[0] for a, b in zip(lv, lv[1:]))
 
 
+@pytest.mark.parametrize("n", [1, 31, 64, 100, 700])
+def test_dense_inverse_spd(n):
+    # blocked Gauss-Jordan with FP64 MFMA updates against numpy's inverse
+    rng = np.random.default_rng(n)
+    m = rng.uniform(-1, 1, (n, n))
+    a = m @ m.T + n * np.eye(n)
+    x = hip.dense_inverse_spd(a)
+    xr = np.linalg.inv(a)
+    assert np.linalg.norm(x - xr) <= 1e-12 * np.linalg.norm(xr)
+    assert np.linalg.norm(a @ x - np.eye(n)) <= 1e-12 * n
+
+
+def test_fixed_cycle_large_coarsest(oracle):
+    # a coarsest level of ~1000 cells: several 32-column Gauss-Jordan blocks
+    s = _sys(4, 20)
+    A = hip.LduMatrix.from_system(s)
+    P = hip.Precond(A, "AMG", amg=hip.amg_params(smoother=2, n_cells_coarsest=1000, n_vcycles=1,
+                                                   n_post_sweeps=4, n_finest_sweeps=4, precision=32, pair_steps=2))
+    assert 600 < P.amg_levels()[-1][0] < 4000
+    x, p = hip.Solver("FCG", P, rel_tol=1e-8, max_iter=300).solve(A, s.b)
+    assert p["converged"]
+    M = oracle.matrix(s)
+    assert np.linalg.norm(s.b - M.spmv(x)) <= 2e-8 * np.linalg.norm(s.b)
+
+
 def test_fp32_hierarchy_needs_fixed_cycle():
     s = _sys(4, 10)
     A = hip.LduMatrix.from_system(s)

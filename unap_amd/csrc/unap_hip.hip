@@ -11,6 +11,7 @@
 #include <functional>
 #include <string>
 
+#include "dense.hpp"
 #include "solve.hpp"
 
 namespace unap_hip {
@@ -222,9 +223,12 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
             }
         }
     } else {
-        // mixed path: dense FP64 inverse of the coarsest Galerkin matrix
+        // fixed cycle: dense FP64 inverse of the coarsest Galerkin matrix,
+        // on the GPU for a symmetric matrix, on the host (partial pivoting)
+        // otherwise
         const int nc = CL.n;
-        if (nc > 8192) throw Error(-4, "coarsest level too large for the dense FP64 solve");
+        if (nc > 16384 || (!CA.symm && nc > 8192))
+            throw Error(-4, "coarsest level too large for the dense FP64 solve");
         std::vector<double> D((size_t)nc * nc, 0.0);
         for (int i = 0; i < nc; ++i) D[(size_t)CL.iperm[i] * nc + CL.iperm[i]] = CA.d[i];
         const double* lw = CA.lower();
@@ -233,9 +237,13 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
             D[(size_t)a * nc + b] += CA.up[f];
             D[(size_t)b * nc + a] += lw[f];
         }
-        std::vector<double> inv = denseInverse(D, nc);
-        Ainv.upload(inv, c.stream);
-        c.sync();
+        if (CA.symm) {
+            denseInverseSPD(D, nc, Ainv, c.stream);
+        } else {
+            std::vector<double> inv = denseInverse(D, nc);
+            Ainv.upload(inv, c.stream);
+            c.sync();
+        }
         // Chebyshev estimates once, here: x = 0 and a fixed pseudo-random b
         if (prm.smoother == UNAP_HIP_SMOOTHER_CHEBY) {
             for (int si = 0; si < L; ++si) {
@@ -919,6 +927,17 @@ int unap_hip_precond_rd(unap_hip_precond P, double* rD) {
         need(P && rD, "null argument");
         if (P->o.rDext.empty()) throw Error(UNAP_HIP_ERR_STATE, "this preconditioner has no rD");
         std::memcpy(rD, P->o.rDext.data(), sizeof(double) * P->o.rDext.size());
+    });
+}
+
+int unap_hip_dense_inverse_spd(int64_t n, const double* a, double* inv) {
+    return guard([&] {
+        need(n > 0 && n <= 16384 && a && inv, "bad dense inverse arguments");
+        std::vector<double> h(a, a + (size_t)n * n);
+        DBuf<double> d;
+        Ctx& c = Ctx::get();
+        denseInverseSPD(h, (int)n, d, c.stream);
+        UH_CHECK(hipMemcpy(inv, d.get(), sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost));
     });
 }
 

@@ -61,6 +61,7 @@ SIGNATURES = {
     "unap_hip_precond_destroy": (ctypes.c_int, [_vp]),
     "unap_hip_precond_apply": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "unap_hip_precond_amg_levels": (ctypes.c_int, [_vp, _i64, _i64p, _i64p, _i64p]),
+    "unap_hip_dense_inverse_spd": (ctypes.c_int, [_i64, _f64p, _f64p]),
     "unap_hip_precond_rd": (ctypes.c_int, [_vp, _f64p]),
     "unap_hip_smoother_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     "unap_hip_smoother_destroy": (ctypes.c_int, [_vp]),
@@ -218,6 +219,15 @@ class DeviceArray:
 def synchronize():
     if _runtime().hipDeviceSynchronize() != 0:
         raise RuntimeError("hipDeviceSynchronize failed")
+
+
+def dense_inverse_spd(a):
+    """Inverse of a dense SPD matrix by the GPU's blocked Gauss-Jordan."""
+    a = np.ascontiguousarray(a, np.float64)
+    n = a.shape[0]
+    out = np.empty((n, n))
+    _check(lib().unap_hip_dense_inverse_spd(n, _p(a), _p(out)))
+    return out
 
 
 def amg_params(**kw):
