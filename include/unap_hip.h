@@ -260,6 +260,9 @@ int unap_hip_batch_solve(unap_hip_batch B, double* x, const double* b, int where
    autotuner chose on the first solve (threads per workgroup, workgroups per CU). */
 int unap_hip_batch_last_ms(unap_hip_batch B, double* ms);
 int unap_hip_batch_config(unap_hip_batch B, int64_t* block_size, int64_t* wg_per_cu);
+/* Set the launch shape and skip the autotuner: block_size 256, 512 or 1024,
+   wg_per_cu >= 1 with block_size * wg_per_cu <= 2048. */
+int unap_hip_batch_set_config(unap_hip_batch B, int64_t block_size, int64_t wg_per_cu);
 
 #ifdef __cplusplus
 }

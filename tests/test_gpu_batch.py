@@ -51,6 +51,25 @@ def test_batch_against_oracle(oracle, nsys, lo, hi, seed):
     assert bs in (256, 512, 1024) and wg in (1, 2, 4)
 
 
+def test_batch_set_config():
+    systems, solvers = _batch(12, 4, 8, 3)
+    B = hip.Batch(systems, solvers)
+    for args in [(128, 1), (256, 0), (1024, 4)]:
+        with pytest.raises(hip.UnapHipError) as e:
+            B.set_config(*args)
+        assert e.value.code == -1
+    b = np.concatenate([s.b for s in systems])
+    B.set_config(256, 2)
+    x1, p1 = B.solve(b, rel_tol=1e-8)
+    assert B.config() == (256, 2)  # no autotuning after set_config
+    B.set_config(1024, 1)
+    x2, p2 = B.solve(b, rel_tol=1e-8)
+    # one workgroup a system either way: the block size changes the order of
+    # the block sums only
+    assert all(p["converged"] for p in p1 + p2)
+    assert np.allclose(x1, x2, rtol=0, atol=1e-6 * np.abs(x1).max())
+
+
 def test_batch_device_pointers_and_repeat():
     systems, solvers = _batch(24, 5, 12, 13)
     B = hip.Batch(systems, solvers)

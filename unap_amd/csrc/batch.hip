@@ -647,4 +647,16 @@ int unap_hip_batch_config(unap_hip_batch B, int64_t* block_size, int64_t* wg_per
     });
 }
 
+int unap_hip_batch_set_config(unap_hip_batch B, int64_t block_size, int64_t wg_per_cu) {
+    return bguard([&] {
+        if (!B) throw Error(UNAP_HIP_ERR_ARG, "null argument");
+        if (!(block_size == 256 || block_size == 512 || block_size == 1024))
+            throw Error(UNAP_HIP_ERR_ARG, "block_size must be 256, 512 or 1024");
+        if (wg_per_cu < 1 || block_size * wg_per_cu > 2048)
+            throw Error(UNAP_HIP_ERR_ARG, "wg_per_cu must be >= 1 with block_size * wg_per_cu <= 2048");
+        B->blockSize = (int)block_size;
+        B->wgPerCU = (int)wg_per_cu;
+    });
+}
+
 }  // extern "C"

@@ -93,6 +93,7 @@ SIGNATURES = {
                                             ctypes.POINTER(Perf)]),
     "unap_hip_batch_last_ms": (ctypes.c_int, [_vp, _f64p]),
     "unap_hip_batch_config": (ctypes.c_int, [_vp, _i64p, _i64p]),
+    "unap_hip_batch_set_config": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64]),
 }
 
 _lib = None
@@ -520,3 +521,7 @@ class Batch:
         bs, wg = _i64(), _i64()
         _check(lib().unap_hip_batch_config(self.h, ctypes.byref(bs), ctypes.byref(wg)))
         return bs.value, wg.value
+
+    def set_config(self, block_size, wg_per_cu):
+        """Fix the launch shape (the autotuner then does not run)."""
+        _check(lib().unap_hip_batch_set_config(self.h, block_size, wg_per_cu))
