@@ -1,0 +1,45 @@
+"""The per-kernel roofline report (unap_amd/kernel_report.py) on a small
+synthetic kernel trace: the load-shape instantiations of one kernel body are
+counted as one kernel, the report starts at the last solve's kStateInit, and
+the per-level table assigns Chebyshev launches coarse to fine."""
+import json
+
+from unap_amd import kernel_report
+
+
+def _trace(tmp_path, names_ns):
+    rows = ["Kernel_Name,Start_Timestamp,End_Timestamp"]
+    t = 0
+    for name, d in names_ns:
+        rows.append(f'"{name}",{t},{t + d}')
+        t += d + 10
+    p = tmp_path / "trace.csv"
+    p.write_text("\n".join(rows) + "\n")
+    return p
+
+
+def test_shapes_merge_and_levels(tmp_path):
+    cheb = "void unap_hip::kRows<unap_hip::BCheby<float, {s}> >(unap_hip::BCheby<float, {s}>, int, unap_hip::Skip)"
+    init = "void unap_hip::kStateInit(unap_hip::KState*)"
+    # an earlier solve that must be ignored, then one V-cycle of a 3-level
+    # hierarchy (2 coarse levels: level 1 smoothed, level 2 coarsest):
+    # 4 sweeps on level 1 (shape 1), then 4 on the finest (shape 0)
+    seq = [(init, 5), (cheb.format(s=0), 999999), (init, 5)]
+    seq += [(cheb.format(s=1), 1000)] * 4 + [(cheb.format(s=0), 4000)] * 4
+    tr = _trace(tmp_path, seq)
+    bench = tmp_path / "bench.json"
+    bench.write_text("progress line\n" + json.dumps({"hierarchy": [[10**8, 3 * 10**8], [25 * 10**6, 9 * 10**7],
+                                                                   [60, 150]],
+                                                     "iterations": 1}) + "\n")
+    out = kernel_report.report(str(tr), str(bench))
+    lines = out.splitlines()
+    row = [ln for ln in lines if ln.startswith("| kRows<BCheby<float> >")]
+    assert len(row) == 1, out
+    cells = [c.strip() for c in row[0].strip("|").split("|")]
+    assert cells[1] == "8"                      # both shapes, one name
+    assert abs(float(cells[2]) - 0.02) < 1e-9   # 4*1000 + 4*4000 ns
+    # algorithmic bytes: 4 sweeps on the finest and on level 1, FP32
+    b = 4 * sum((6 * 4 + 4) * n + (2 * 4 + 8) * f for n, f in [(10**8, 3 * 10**8), (25 * 10**6, 9 * 10**7)])
+    assert abs(float(cells[4]) - b / 1e9) < 1e-2
+    lvl = {ln.split("|")[1].strip(): ln for ln in lines if ln.startswith("| ") and ln.split("|")[1].strip().isdigit()}
+    assert "| 4.0 |" in lvl["0"] and "| 1.0 |" in lvl["1"], out

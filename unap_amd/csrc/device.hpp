@@ -143,6 +143,7 @@ struct DevMatrix {
     DBuf<int> fwdRows, fwdStart, bwdRows, bwdStart;  // fwdRows empty: level-major order
     std::vector<int> fwdStartH, bwdStartH;
     bool sweepable = false;
+    int shape = 0;  // rowAmul load shape of the V-cycle kernels: kShapeBatch4 above 14 entries a row
 
     DMat<T> view() const { return {n, sliceStart.get(), rowInfo.get(), col.get(), val.get(), diag.get()}; }
     DMat<T> viewU() const {
@@ -152,6 +153,7 @@ struct DevMatrix {
     void build(const HostLdu& A, const Layout& L, hipStream_t s) {
         n = L.n;
         sweepable = L.sweepable;
+        shape = (2.0 * L.nf > 14.0 * L.n) ? kShapeBatch4 : kShapeWide8;
         sliceStart.upload(L.sliceStart, s);
         rowInfo.upload(L.rowInfo, s);
         col.upload(L.col, s);

@@ -66,12 +66,21 @@ struct Skip {
 };
 
 // acc = 0 + d*x[r]; acc += v_k * x[c_k] over the row's faces in order.
-// Entries are fetched kUnroll at a time (coefficients and columns first, then
-// the x gathers) so each lane keeps several loads in flight; the additions
-// stay in entry order.  On a 256^3 FP32 matrix this lifts Amul from 3.8 to
-// 4.6 TB/s.
-constexpr int kUnroll = 4;
-template <typename T, typename TA, typename TX>
+// Entries are fetched in batches (the batch's columns and coefficients first,
+// then its x gathers) so each lane keeps several loads in flight; the
+// additions stay in entry order.  Two load shapes, chosen per matrix at
+// compile time of the launching kernel:
+//  * kShapeWide8 (default): batches of 8 whose positions past the slice width
+//    load nothing.  Best where rows carry up to ~12 entries: at 256^3 the
+//    finest FP32 Chebyshev sweep 263 -> 255 us, the FP64 Amul 406 -> 360 us,
+//    the first coarse level (12 entries a row) 150 -> 145 us;
+//  * kShapeBatch4: batches of 4 and a scalar tail, better for the longer rows
+//    (17-23 entries) of the smaller coarse levels, whose launches are
+//    latency-bound (e.g. 198k rows: 14.7 us against 20.7 us).
+constexpr int kUnroll = 4;  // also the load batch of a sweep row (rowSub)
+constexpr int kAmulBatch = 8;
+constexpr int kShapeWide8 = 0, kShapeBatch4 = 1;
+template <typename T, typename TA, typename TX, int Shape = kShapeWide8>
 __device__ __forceinline__ TA rowAmul(const DMat<T>& A, int r, const TX* __restrict__ x) {
     const int s = r >> kSliceShift;
     const long long s0 = A.sliceStart[s];
@@ -79,28 +88,47 @@ __device__ __forceinline__ TA rowAmul(const DMat<T>& A, int r, const TX* __restr
     const int len = (int)(A.rowInfo[r] & 0xffffu);
     const long long base = s0 + (r & (kSliceRows - 1));
     TA acc = (TA)0 + (TA)A.diag[r] * (TA)x[r];
-    int k = 0;
-    for (; k + kUnroll <= slen; k += kUnroll) {
-        int c[kUnroll];
-        T v[kUnroll];
-        TX xv[kUnroll];
+    if constexpr (Shape == kShapeWide8) {
+        for (int k = 0; k < slen; k += kAmulBatch) {
+            int c[kAmulBatch];
+            T v[kAmulBatch];
+            TX xv[kAmulBatch];
 #pragma unroll
-        for (int j = 0; j < kUnroll; ++j) {
-            const long long p = base + ((long long)(k + j) << kSliceShift);
-            c[j] = A.col[p];
-            v[j] = A.val[p];
+            for (int j = 0; j < kAmulBatch; ++j) {
+                const long long p = base + ((long long)(k + j) << kSliceShift);
+                c[j] = (k + j < slen) ? A.col[p] : 0;
+                v[j] = (k + j < slen) ? A.val[p] : (T)0;
+            }
+#pragma unroll
+            for (int j = 0; j < kAmulBatch; ++j) xv[j] = (k + j < slen) ? x[c[j]] : (TX)0;
+#pragma unroll
+            for (int j = 0; j < kAmulBatch; ++j)
+                if (k + j < len) acc = acc + (TA)v[j] * (TA)xv[j];
         }
+    } else {
+        int k = 0;
+        for (; k + kUnroll <= slen; k += kUnroll) {
+            int c[kUnroll];
+            T v[kUnroll];
+            TX xv[kUnroll];
 #pragma unroll
-        for (int j = 0; j < kUnroll; ++j) xv[j] = x[c[j]];
+            for (int j = 0; j < kUnroll; ++j) {
+                const long long p = base + ((long long)(k + j) << kSliceShift);
+                c[j] = A.col[p];
+                v[j] = A.val[p];
+            }
 #pragma unroll
-        for (int j = 0; j < kUnroll; ++j)
-            if (k + j < len) acc = acc + (TA)v[j] * (TA)xv[j];
-    }
-    for (; k < slen; ++k) {
-        const long long p = base + ((long long)k << kSliceShift);
-        const TA v = (TA)A.val[p];
-        const TA xv = (TA)x[A.col[p]];
-        if (k < len) acc = acc + v * xv;
+            for (int j = 0; j < kUnroll; ++j) xv[j] = x[c[j]];
+#pragma unroll
+            for (int j = 0; j < kUnroll; ++j)
+                if (k + j < len) acc = acc + (TA)v[j] * (TA)xv[j];
+        }
+        for (; k < slen; ++k) {
+            const long long p = base + ((long long)k << kSliceShift);
+            const TA v = (TA)A.val[p];
+            const TA xv = (TA)x[A.col[p]];
+            if (k < len) acc = acc + v * xv;
+        }
     }
     return acc;
 }

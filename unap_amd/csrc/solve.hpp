@@ -146,11 +146,23 @@ void chebySweeps(DevMatrix<T>& M, T* x, const T* b, int nSweeps, const KState* e
     T* cur = x;
     T* nxt = tmp;
     for (int s = 0; s < nSweeps; ++s) {
-        launchRows(BCheby<T>{M.view(), cur, nxt, b, w, est, s}, M.n, sk);
+        if (M.shape == kShapeBatch4)
+            launchRows(BCheby<T, kShapeBatch4>{M.view(), cur, nxt, b, w, est, s}, M.n, sk);
+        else
+            launchRows(BCheby<T>{M.view(), cur, nxt, b, w, est, s}, M.n, sk);
         std::swap(cur, nxt);
     }
     if (cur != x)
         UH_CHECK(hipMemcpyAsync(x, cur, sizeof(T) * M.n, hipMemcpyDeviceToDevice, Ctx::get().stream));
+}
+
+// Acf = A f with the sums of scalingFactor, in the matrix's rowAmul shape.
+template <typename T, typename Fin>
+void sfAmul(DevMatrix<T>& M, const T* f, const T* src, T* Acf, Fin fin, Skip sk) {
+    if (M.shape == kShapeBatch4)
+        launchReduce<2>(BSfAmul<T, kShapeBatch4>{M.view(), f, src, Acf}, M.n, M.iperm.get(), fin, sk);
+    else
+        launchReduce<2>(BSfAmul<T>{M.view(), f, src, Acf}, M.n, M.iperm.get(), fin, sk);
 }
 
 // eigenDiagPCG (eigenDiagPCG.cpp:41-212): nPCG Jacobi-PCG steps updating x,

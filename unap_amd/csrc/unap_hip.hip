@@ -341,8 +341,7 @@ void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, bool psiZero, Skip s
             launchRows(BFill<T>{W.a.get(), (T)0}, n, sk);
             smooth(l + 1, W.a.get(), W.b.get(), prm.nPre + prm.growth * l, sk);
             if (scale && l < coarsest - 1) {
-                launchReduce<2>(BSfAmul<T>{mat(l).view(), W.a.get(), W.b.get(), W.c.get()}, n,
-                                mat(l).iperm.get(), FinSF{mgst.get(), l}, sk);
+                sfAmul(mat(l), W.a.get(), W.b.get(), W.c.get(), FinSF{mgst.get(), l}, sk);
                 launchRows(BJacobiScale<T>{W.a.get(), W.b.get(), W.c.get(), mat(l).diag.get(), mgst.get(), l,
                                            nullptr},
                            n, sk);
@@ -359,8 +358,7 @@ void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, bool psiZero, Skip s
         if (prm.nPre) launchRows(BCopy<T, T>{W.a.get(), pre[l].get()}, n, sk);
         launchRows(BProlong<T>{f2c[l + 1].get(), work[l + 1].a.get(), W.a.get()}, n, sk);
         if (scale && l < coarsest - 1) {
-            launchReduce<2>(BSfAmul<T>{mat(l).view(), W.a.get(), W.b.get(), W.c.get()}, n,
-                            mat(l).iperm.get(), FinSF{mgst.get(), l}, sk);
+            sfAmul(mat(l), W.a.get(), W.b.get(), W.c.get(), FinSF{mgst.get(), l}, sk);
             launchRows(BJacobiScale<T>{W.a.get(), W.b.get(), W.c.get(), mat(l).diag.get(), mgst.get(), l,
                                        prm.nPre ? pre[l].get() : nullptr},
                        n, sk);
@@ -372,7 +370,7 @@ void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, bool psiZero, Skip s
     const int n = fine->n;
     launchRows(BProlong<T>{f2c[0].get(), work[0].a.get(), fw.a.get()}, n, sk);
     if (scale) {
-        launchReduce<2>(BSfAmul<T>{fine->view(), fw.a.get(), fRes, fw.b.get()}, n, fine->iperm.get(), FinSF{mgst.get(), 63}, sk);
+        sfAmul(*fine, fw.a.get(), fRes, fw.b.get(), FinSF{mgst.get(), 63}, sk);
         launchRows(BPsiScaled<T>{psi, fw.a.get(), fRes, fw.b.get(), fine->diag.get(), mgst.get(), 63, psiZero}, n,
                    sk);
     } else if (psiZero) {

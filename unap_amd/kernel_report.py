@@ -67,6 +67,12 @@ def alg_bytes(kernel, lv, iters):
     return None
 
 
+def normalize(name):
+    """One name for the load-shape instantiations of a kernel body
+    (BCheby<float, 0> and BCheby<float, 1> -> BCheby<float>)."""
+    return re.sub(r"<(float|double), \d+>", r"<\1>", name)
+
+
 def short(name):
     name = name.replace("unap_hip::", "")
     m = re.match(r"void (\w+)<(.*)>\(", name)
@@ -75,6 +81,8 @@ def short(name):
 
 def report(trace_csv, bench_json):
     rows = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
+    for r in rows:
+        r["Kernel_Name"] = normalize(r["Kernel_Name"])
     starts = [i for i, r in enumerate(rows) if "kStateInit" in r["Kernel_Name"]]
     if not starts:
         raise SystemExit("no kStateInit dispatch in the trace")
