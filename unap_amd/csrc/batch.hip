@@ -27,12 +27,28 @@ namespace unap_hip {
 
 struct BatchSys {
     int n, nf, type, nFwd, nBwd;
-    int cellOff;            // cells / rows / internal vectors
+    int cellOff;            // cells / rows / internal vectors / sweep positions
     long long entOff;       // SELL entries
     int sliceOff;           // slice starts (n_slices + 1 per system)
     int fwdOff, bwdOff;     // level boundaries
-    int levelMajor;         // forward levels are contiguous internal rows
+    long long tabOff[2];    // sweep-table entries (forward, backward)
+    int tabM[2];            // entry slots per position (forward, backward)
 };
+
+// A preconditioner sweep in table form.  Positions p = 0 .. n-1 follow the
+// sweep's level order; level k holds positions [start[k], start[k+1]).
+// Position p names its row and the number of sweep-side entries (row | cnt <<
+// 24), the row's rD, and entry j's column and rD * coefficient (FP32 product,
+// the one the sweep forms) at j * n + p.  Every address follows from p alone,
+// so the next level's entries are loaded before the level barrier.
+struct SweepTab {
+    const uint32_t* rc;
+    const float* rd;
+    const int* col;
+    const float* coef;
+    int n, m;
+};
+constexpr int kSweepPre = 4;  // entries a thread holds for its next row
 
 struct BatchArgs {
     const BatchSys* sys;
@@ -44,12 +60,12 @@ struct BatchArgs {
     const int* col;
     const double* val;
     const double* diag;
-    const float* valP;  // preconditioner coefficients (DIC: upper both sides; DILU: lower / upper)
-    const float* rD;
     const int* fwdStart;
-    const int* fwdRows;
-    const int* bwdRows;
     const int* bwdStart;
+    const uint32_t* tabRc[2];  // sweep tables (forward, backward)
+    const float* tabRd[2];
+    const int* tabCol[2];
+    const float* tabCoef[2];
     const double* b;
     double* x;
     double* v;  // 8 FP64 work vectors, each of total length
@@ -95,36 +111,78 @@ __device__ __forceinline__ void bsum2(double& v0, double& v1, double* red) {
 
 struct SysView {
     DMat<double> A;
-    DMat<float> P;
-    const float* rD;
-    const int* frows;  // null: forward levels are contiguous
+    SweepTab fwd, bwd;
     const int* fs;
     const int* bs;
-    const int* brows;
     int nFwd, nBwd, n;
 };
+
+struct SweepRow {
+    int q, cnt;
+    float rd;
+    int c[kSweepPre];
+    float v[kSweepPre];
+};
+
+__device__ __forceinline__ void loadSweepRow(SweepRow& R, const SweepTab& T, int p) {
+    const uint32_t rc = T.rc[p];
+    R.q = (int)(rc & 0xffffffu);
+    R.cnt = (int)(rc >> 24);
+    R.rd = T.rd[p];
+#pragma unroll
+    for (int j = 0; j < kSweepPre; ++j)
+        if (j < T.m) {
+            R.c[j] = T.col[(size_t)j * T.n + p];
+            R.v[j] = T.coef[(size_t)j * T.n + p];
+        }
+}
+
+// acc -= coef_j * w[col_j] in entry order (the DIC/DILU row arithmetic of
+// fwdRow / bwdRow: coef_j is the FP32 product rD * coefficient).
+__device__ __forceinline__ float sweepRow(const SweepRow& R, const SweepTab& T, int p, float acc, const float* w) {
+    float x[kSweepPre];
+#pragma unroll
+    for (int j = 0; j < kSweepPre; ++j)
+        if (j < T.m) x[j] = w[R.c[j]];
+#pragma unroll
+    for (int j = 0; j < kSweepPre; ++j)
+        if (j < R.cnt) acc = acc - R.v[j] * x[j];
+    for (int j = kSweepPre; j < R.cnt; ++j) acc = acc - T.coef[(size_t)j * T.n + p] * w[T.col[(size_t)j * T.n + p]];
+    return acc;
+}
+
+// One sweep over the levels [start[0], start[nl]).  Forward rows start from
+// rD * r (r read in FP64), backward rows from their forward value.
+template <bool Fwd>
+__device__ __forceinline__ void batchSweep(const SweepTab& T, const int* start, int nl, const double* r, float* w) {
+    const int nt = (int)blockDim.x;
+    SweepRow R;
+    int p = start[0] + (int)threadIdx.x;
+    if (nl > 0 && p < start[1]) loadSweepRow(R, T, p);
+    for (int k = 0; k < nl; ++k) {
+        const int e = start[k + 1];
+        const int pn = e + (int)threadIdx.x;
+        SweepRow Rn;
+        if (k + 1 < nl && pn < start[k + 2]) loadSweepRow(Rn, T, pn);
+        if (p < e) {
+            w[R.q] = sweepRow(R, T, p, Fwd ? R.rd * (float)r[R.q] : w[R.q], w);
+            for (int p2 = p + nt; p2 < e; p2 += nt) {
+                SweepRow R2;
+                loadSweepRow(R2, T, p2);
+                w[R2.q] = sweepRow(R2, T, p2, Fwd ? R2.rd * (float)r[R2.q] : w[R2.q], w);
+            }
+        }
+        __syncthreads();
+        R = Rn;
+        p = pn;
+    }
+}
 
 // w32 = M^-1 r in FP32 with the level schedule (DIC or DILU arithmetic of
 // fwdRow / bwdRow, here reading r in FP64).
 __device__ __forceinline__ void batchPrecond(const SysView& s, const double* r, float* w) {
-    for (int k = 0; k < s.nFwd; ++k) {
-        for (int qq = s.fs[k] + threadIdx.x; qq < s.fs[k + 1]; qq += blockDim.x) {
-            const int q = s.frows ? s.frows[qq] : qq;
-            RowEnt<float> e(s.P, q);
-            const float rd = s.rD[q];
-            w[q] = rowSub<true, float, float, float>(s.P, e, 0, e.mid(), 1, rd * (float)r[q], rd, w);
-        }
-        __syncthreads();
-    }
-    for (int k = 0; k < s.nBwd; ++k) {
-        for (int q = s.bs[k] + threadIdx.x; q < s.bs[k + 1]; q += blockDim.x) {
-            const int row = s.brows[q];
-            RowEnt<float> e(s.P, row);
-            const float rd = s.rD[row];
-            w[row] = rowSub<true, float, float, float>(s.P, e, e.len() - 1, e.mid() - 1, -1, w[row], rd, w);
-        }
-        __syncthreads();
-    }
+    batchSweep<true>(s.fwd, s.fs, s.nFwd, r, w);
+    batchSweep<false>(s.bwd, s.bs, s.nBwd, r, w);
 }
 
 // Shadow-residual restart threshold of the FP32-preconditioned BiCGStab.
@@ -345,13 +403,12 @@ __global__ void __launch_bounds__(BS) kBatchSolve(BatchArgs a) {
         s.n = d.n;
         s.A = {d.n, a.sliceStart + d.sliceOff, a.rowInfo + d.cellOff, a.col + d.entOff, a.val + d.entOff,
                a.diag + d.cellOff};
-        s.P = {d.n, a.sliceStart + d.sliceOff, a.rowInfo + d.cellOff, a.col + d.entOff, a.valP + d.entOff,
-               nullptr};
-        s.rD = a.rD + d.cellOff;
+        SweepTab* tabs[2] = {&s.fwd, &s.bwd};
+        for (int dir = 0; dir < 2; ++dir)
+            *tabs[dir] = {a.tabRc[dir] + d.cellOff, a.tabRd[dir] + d.cellOff, a.tabCol[dir] + d.tabOff[dir],
+                          a.tabCoef[dir] + d.tabOff[dir], d.n, d.tabM[dir]};
         s.fs = a.fwdStart + d.fwdOff;
-        s.frows = d.levelMajor ? nullptr : a.fwdRows + d.cellOff;
         s.bs = a.bwdStart + d.bwdOff;
-        s.brows = a.bwdRows + d.cellOff;
         s.nFwd = d.nFwd;
         s.nBwd = d.nBwd;
         if (d.type == UNAP_HIP_SOLVER_PCG)
@@ -380,9 +437,12 @@ struct unap_hip_batch_s {
     DBuf<unsigned> ticket;
     DBuf<long long> sliceStart;
     DBuf<uint32_t> rowInfo;
-    DBuf<int> col, fwdStart, fwdRows, bwdRows, bwdStart, perm;
+    DBuf<int> col, fwdStart, bwdStart, perm;
     DBuf<double> val, diag, x, b, v;
-    DBuf<float> valP, rD, w32;
+    DBuf<float> w32;
+    DBuf<uint32_t> tabRc[2];
+    DBuf<float> tabRd[2], tabCoef[2];
+    DBuf<int> tabCol[2];
     DBuf<unap_hip_perf> perf;
     int blockSize = 0, wgPerCU = 0;  // 0 = not tuned yet
     double lastMs = 0;
@@ -418,11 +478,13 @@ void launchBatch(unap_hip_batch_s& B, double relTol, double tol, long long maxIt
     a.col = B.col.get();
     a.val = B.val.get();
     a.diag = B.diag.get();
-    a.valP = B.valP.get();
-    a.rD = B.rD.get();
+    for (int dir = 0; dir < 2; ++dir) {
+        a.tabRc[dir] = B.tabRc[dir].get();
+        a.tabRd[dir] = B.tabRd[dir].get();
+        a.tabCol[dir] = B.tabCol[dir].get();
+        a.tabCoef[dir] = B.tabCoef[dir].get();
+    }
     a.fwdStart = B.fwdStart.get();
-    a.fwdRows = B.fwdRows.get();
-    a.bwdRows = B.bwdRows.get();
     a.bwdStart = B.bwdStart.get();
     a.b = B.b.get();
     a.x = B.x.get();
@@ -460,14 +522,17 @@ int unap_hip_batch_create(int64_t nsys, const int64_t* cellOff, const int64_t* f
         B->nsys = (int)nsys;
         std::vector<long long> sliceAll;
         std::vector<uint32_t> infoAll;
-        std::vector<int> colAll, fwdAll, fwdRowsAll, bwdRowsAll, bwdAll, permAll;
+        std::vector<int> colAll, fwdAll, bwdAll, permAll;
         std::vector<double> valAll, diagAll;
-        std::vector<float> valPAll, rDAll;
+        std::vector<uint32_t> tRc[2];
+        std::vector<float> tRd[2], tCoef[2];
+        std::vector<int> tCol[2];
         long long ent = 0;
         std::vector<double> work(nsys);
         for (int64_t s = 0; s < nsys; ++s) {
             const int64_t n = cellOff[s + 1] - cellOff[s], nf = faceOff[s + 1] - faceOff[s];
             if (n <= 0 || nf < 0) throw Error(UNAP_HIP_ERR_ARG, "empty system in batch");
+            if (n >= (1 << 24)) throw Error(UNAP_HIP_ERR_ARG, "batch systems hold fewer than 2^24 cells");
             if (solver[s] != UNAP_HIP_SOLVER_PCG && solver[s] != UNAP_HIP_SOLVER_PBICGSTAB)
                 throw Error(UNAP_HIP_ERR_ARG, "batch systems use PCG (DIC) or PBiCGStab (DILU)");
             HostLdu A;
@@ -498,32 +563,69 @@ int unap_hip_batch_create(int64_t nsys, const int64_t* cellOff, const int64_t* f
             d.sliceOff = (int)sliceAll.size();
             d.fwdOff = (int)fwdAll.size();
             d.bwdOff = (int)bwdAll.size();
-            d.levelMajor = L.fwdRows.empty() ? 1 : 0;
+            const bool dic = (solver[s] == UNAP_HIP_SOLVER_PCG);
+            std::vector<float> vp;
+            fillSellValues<float>(L, A, dic, vp);
+            const std::vector<double> rde = dic ? dicReciprocalD(A) : diluReciprocalD(A);
+            std::vector<float> rd32(n);
+            for (int i = 0; i < (int)n; ++i) rd32[i] = (float)rde[L.perm[i]];
+            // sweep tables: forward over the level-major rows, entries in
+            // losort order; backward over bwdRows, owner entries descending
+            for (int dir = 0; dir < 2; ++dir) {
+                auto rowAt = [&](int p) {
+                    if (dir == 1) return L.bwdRows[p];
+                    return L.fwdRows.empty() ? p : L.fwdRows[p];
+                };
+                auto entries = [&](int q, int& k0, int& cnt, int& step) {
+                    const int len = (int)(L.rowInfo[q] & 0xffffu), mid = (int)(L.rowInfo[q] >> 16);
+                    k0 = dir == 0 ? 0 : len - 1;
+                    cnt = dir == 0 ? mid : len - mid;
+                    step = dir == 0 ? 1 : -1;
+                };
+                int M = 0;
+                for (int p = 0; p < (int)n; ++p) {
+                    int k0, cnt, step;
+                    entries(rowAt(p), k0, cnt, step);
+                    M = std::max(M, cnt);
+                }
+                if (M > 255) throw Error(UNAP_HIP_ERR_ARG, "batch rows hold at most 255 faces a side");
+                d.tabOff[dir] = (long long)tCol[dir].size();
+                d.tabM[dir] = M;
+                const size_t o = tCol[dir].size();
+                tCol[dir].resize(o + (size_t)M * n);
+                tCoef[dir].resize(o + (size_t)M * n, 0.f);
+                for (int p = 0; p < (int)n; ++p) {
+                    const int q = rowAt(p);
+                    int k0, cnt, step;
+                    entries(q, k0, cnt, step);
+                    tRc[dir].push_back((uint32_t)q | ((uint32_t)cnt << 24));
+                    tRd[dir].push_back(rd32[q]);
+                    const long long base = L.sliceStart[q >> kSliceShift] + (q & (kSliceRows - 1));
+                    for (int j = 0; j < M; ++j) {
+                        const size_t at = o + (size_t)j * n + p;
+                        if (j < cnt) {
+                            const long long e = base + ((long long)(k0 + j * step) << kSliceShift);
+                            tCol[dir][at] = L.col[e];
+                            tCoef[dir][at] = rd32[q] * vp[e];  // the FP32 product the sweep forms
+                        } else {
+                            tCol[dir][at] = q;  // padding: a valid address, never added
+                        }
+                    }
+                }
+            }
             B->sysH.push_back(d);
             sliceAll.insert(sliceAll.end(), L.sliceStart.begin(), L.sliceStart.end());
             infoAll.insert(infoAll.end(), L.rowInfo.begin(), L.rowInfo.end());
             colAll.insert(colAll.end(), L.col.begin(), L.col.end());
             fwdAll.insert(fwdAll.end(), L.fwdStart.begin(), L.fwdStart.end());
             bwdAll.insert(bwdAll.end(), L.bwdStart.begin(), L.bwdStart.end());
-            bwdRowsAll.insert(bwdRowsAll.end(), L.bwdRows.begin(), L.bwdRows.end());
-            if (L.fwdRows.empty()) {
-                for (int i = 0; i < (int)n; ++i) fwdRowsAll.push_back(i);
-            } else {
-                fwdRowsAll.insert(fwdRowsAll.end(), L.fwdRows.begin(), L.fwdRows.end());
-            }
             for (int i = 0; i < (int)n; ++i) permAll.push_back((int)cellOff[s] + L.perm[i]);
             std::vector<double> v;
             fillSellValues<double>(L, A, false, v);
             valAll.insert(valAll.end(), v.begin(), v.end());
-            const bool dic = (solver[s] == UNAP_HIP_SOLVER_PCG);
-            std::vector<float> vp;
-            fillSellValues<float>(L, A, dic, vp);
-            valPAll.insert(valPAll.end(), vp.begin(), vp.end());
             std::vector<double> dd;
             fillDiag<double>(L, A, dd);
             diagAll.insert(diagAll.end(), dd.begin(), dd.end());
-            const std::vector<double> rde = dic ? dicReciprocalD(A) : diluReciprocalD(A);
-            for (int i = 0; i < (int)n; ++i) rDAll.push_back((float)rde[L.perm[i]]);
             ent += L.nnzPadded;
             work[s] = (double)(n + 2 * nf) * std::cbrt((double)n) * (dic ? 1.0 : 2.0);
         }
@@ -539,13 +641,19 @@ int unap_hip_batch_create(int64_t nsys, const int64_t* cellOff, const int64_t* f
         B->col.upload(colAll, c.stream);
         B->fwdStart.upload(fwdAll, c.stream);
         B->bwdStart.upload(bwdAll, c.stream);
-        B->bwdRows.upload(bwdRowsAll, c.stream);
-        B->fwdRows.upload(fwdRowsAll, c.stream);
+        for (int dir = 0; dir < 2; ++dir) {
+            if (tCol[dir].empty()) {  // diagonal systems only: keep the pointers valid
+                tCol[dir].push_back(0);
+                tCoef[dir].push_back(0.f);
+            }
+            B->tabRc[dir].upload(tRc[dir], c.stream);
+            B->tabRd[dir].upload(tRd[dir], c.stream);
+            B->tabCol[dir].upload(tCol[dir], c.stream);
+            B->tabCoef[dir].upload(tCoef[dir], c.stream);
+        }
         B->perm.upload(permAll, c.stream);
         B->val.upload(valAll, c.stream);
-        B->valP.upload(valPAll, c.stream);
         B->diag.upload(diagAll, c.stream);
-        B->rD.upload(rDAll, c.stream);
         B->x.alloc(B->total);
         B->b.alloc(B->total);
         B->v.alloc(8 * B->total);
