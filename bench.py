@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_256.json"),
                     help="JSON with the HBM bytes per launch of the dominant kernel, measured by rocprofv3 --pmc")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-systems section")
+    ap.add_argument("--no-configs", action="store_true", help="skip the C2 / C3 FP64 drop-in solves")
+    ap.add_argument("--config-size", type=int, default=128, help="edge length of the C2 / C3 systems")
     ap.add_argument("--batch-launches", type=int, default=20)
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--kernel-only", default=None,
@@ -194,9 +196,13 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_size or N, p64["nIterations"], args.n_coarsest)
 
+    del P32, P64, P64f, S32, S64, S64f, Sref, A, x, b
+    configs = None
+    if not args.no_configs:
+        configs = config_legs(args.config_size, not args.no_cpu)
+
     batch = None
     if not args.no_batch:
-        del P32, P64, P64f, S32, S64, S64f, Sref, A, x, b
         batch = batch_bench(args.batch_systems, args.batch_launches, not args.no_cpu)
 
     t32 = float(np.median(times32))
@@ -223,9 +229,68 @@ def main():
         "setup_s": {"generate": round(t_gen, 2), "ldu_create": round(t_create, 2),
                     "mixed_hierarchy": round(t_setup32, 2), "fp64_hierarchy": round(t_setup64, 2)},
         "device": dev_name, "roofline": roofline, "kernels": kernels, "cpu_baseline": cpu,
-        "batch": batch,
+        "configs": configs, "batch": batch,
     }
     print(json.dumps(out), flush=True)
+
+
+def config_legs(N, with_cpu, reps=3):
+    """The FP64 drop-in configurations at their own sizes: C2 (PCG + AMG with
+    Gauss-Seidel smoothing, coarsest level <= 500 cells) and C3 (PBiCGStab +
+    DILU), rel. tol 1e-8, each timed on the GPU (median of `reps` solves from
+    x = 0, setup excluded) beside the reference's x86 solve of the same system
+    (one and three iterations timed, extrapolated to the GPU's count)."""
+    from unap_amd import hip, meshgen
+
+    legs = {
+        "c2": ("PCG", "AMG", {"smoother": 1, "n_cells_coarsest": 500, "n_vcycles": 1}, [1, 500, 1, 0, 2, 2],
+               "PCG + FP64 AMG, Gauss-Seidel V(0,2), coarsest <= 500 cells"),
+        "c3": ("PBiCGStab", "DILU", None, None, "PBiCGStab + DILU"),
+    }
+    out = {}
+    for key, (solver, pre, amg, mg, what) in legs.items():
+        s = meshgen.config_system(int(key[1]), N)
+        A = hip.LduMatrix.from_system(s)
+        t0 = time.time()
+        P = hip.Precond(A, pre, amg=hip.amg_params(**amg) if amg else None)
+        S = hip.Solver(solver, P, rel_tol=1e-8, max_iter=1000)
+        b = hip.DeviceArray.from_numpy(s.b)
+        x = hip.DeviceArray(s.n)
+        _, perf = S.solve(A, b, x=x)  # warm-up (first-call estimates)
+        t_setup = time.time() - t0
+        times = []
+        for _ in range(reps):
+            x.zero()
+            hip.synchronize()
+            t1 = time.perf_counter()
+            _, perf = S.solve(A, b, x=x)
+            hip.synchronize()
+            times.append(time.perf_counter() - t1)
+        t = float(np.median(times))
+        res = _ldu_residual(s, x.numpy())
+        leg = {"workload": f"{key.upper()}: {N}^3 {what}", "s": round(t, 5), "iterations": perf["nIterations"],
+               "converged": perf["converged"], "true_rel_residual": res, "setup_and_first_solve_s": round(t_setup, 2)}
+        log(f"{key}: {t * 1e3:.1f} ms, {perf['nIterations']} iterations, converged={perf['converged']}, "
+            f"true rel. residual {res:.2e}")
+        if with_cpu:
+            from oracle import oracle as O
+            path = O.reference()
+            kind = "reference"
+            if path is None:
+                path, kind = O.oracle(), "port"
+            M = path.matrix(s)
+            _, _, _, t_one = O.timed_solve(path, M, solver, pre, s.b, 1, 1, 1e-8, mg=mg)
+            _, _, _, t_three = O.timed_solve(path, M, solver, pre, s.b, 1, 3, 1e-8, mg=mg)
+            per_it = (t_three - t_one) / 2
+            est = t_one + (perf["nIterations"] - 1) * per_it
+            leg["cpu_baseline"] = {"value": round(est, 2), "unit": "s", "cores": 1, "kind": kind,
+                                   "sample": f"solves of 1 and 3 iterations ({t_one:.2f}s, {t_three:.2f}s) after a "
+                                             f"cold one, extrapolated to {perf['nIterations']} iterations"}
+            leg["speedup_vs_cpu"] = round(est / t, 1)
+            log(f"{key} cpu ({kind}): {per_it:.2f}s an iteration -> {est:.1f}s")
+        out[key] = leg
+        del S, P, A, b, x
+    return out
 
 
 def _ldu_residual(s, x):

@@ -128,24 +128,42 @@ struct CoarseLocal {
     const int* frows;
     const int* bs_;
     const int* brows;
-    __device__ double amul(int i, const double* x) const {
-        double acc = 0.0 + diag[i] * x[i];
-        for (int k = start[i]; k < start[i + 1]; ++k) acc = acc + val[k] * x[col[k]];
+    // acc (+ or - rd *) val[k] * x[col[k]] over k = k0, k0 + step, ... (n
+    // entries) in that order.  LDS loads go kLocalBatch at a time (columns
+    // and coefficients, then the gathers), positions past the end re-reading
+    // the batch's first entry, so a row waits on two LDS round trips a batch
+    // rather than two an entry.
+    static constexpr int kLocalBatch = 8;
+    template <int Op>  // 0: acc + v x;  1: acc - (rd v) x
+    __device__ __forceinline__ double run(int k0, int n, int step, double acc, double rd, const double* x) const {
+        for (int t0 = 0; t0 < n; t0 += kLocalBatch) {
+            int c[kLocalBatch];
+            double v[kLocalBatch], xv[kLocalBatch];
+#pragma unroll
+            for (int j = 0; j < kLocalBatch; ++j) {
+                const int k = k0 + step * (t0 + j < n ? t0 + j : t0);
+                c[j] = col[k];
+                v[j] = val[k];
+            }
+#pragma unroll
+            for (int j = 0; j < kLocalBatch; ++j) xv[j] = x[c[j]];
+#pragma unroll
+            for (int j = 0; j < kLocalBatch; ++j)
+                if (t0 + j < n) acc = Op == 0 ? acc + v[j] * xv[j] : acc - (rd * v[j]) * xv[j];
+        }
         return acc;
+    }
+    __device__ double amul(int i, const double* x) const {
+        return run<0>(start[i], start[i + 1] - start[i], 1, 0.0 + diag[i] * x[i], 0.0, x);
     }
     __device__ void fwd(int i, double* w, const double* r) const {
         const double rd = rD[i];
-        double acc = rd * r[i];
-        const int e = start[i] + mid[i];
-        for (int k = start[i]; k < e; ++k) acc = acc - (rd * val[k]) * w[col[k]];
-        w[i] = acc;
+        w[i] = run<1>(start[i], mid[i], 1, rd * r[i], rd, w);
     }
     __device__ void bwd(int i, double* w) const {
         const double rd = rD[i];
-        double acc = w[i];
         const int b = start[i] + mid[i];
-        for (int k = start[i + 1] - 1; k >= b; --k) acc = acc - (rd * val[k]) * w[col[k]];
-        w[i] = acc;
+        w[i] = run<1>(start[i + 1] - 1, start[i + 1] - b, -1, w[i], rd, w);
     }
     __device__ int fs(int k) const { return fs_[k]; }
     __device__ int frow(int q) const { return frows ? frows[q] : q; }

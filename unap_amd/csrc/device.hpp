@@ -144,6 +144,7 @@ struct DevMatrix {
     std::vector<int> fwdStartH, bwdStartH;
     bool sweepable = false;
     int shape = 0;  // rowAmul load shape of the V-cycle kernels: kShapeBatch4 above 14 entries a row
+    double entriesPerRow = 0;  // off-diagonal entries a row, on average
 
     DMat<T> view() const { return {n, sliceStart.get(), rowInfo.get(), col.get(), val.get(), diag.get()}; }
     DMat<T> viewU() const {
@@ -154,6 +155,7 @@ struct DevMatrix {
         n = L.n;
         sweepable = L.sweepable;
         shape = (2.0 * L.nf > 14.0 * L.n) ? kShapeBatch4 : kShapeWide8;
+        entriesPerRow = L.n > 0 ? 2.0 * L.nf / L.n : 0.0;
         sliceStart.upload(L.sliceStart, s);
         rowInfo.upload(L.rowInfo, s);
         col.upload(L.col, s);

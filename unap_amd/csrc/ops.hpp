@@ -496,35 +496,29 @@ struct RowEnt {
 template <bool Scaled, typename T, typename TV, typename TX>
 __device__ __forceinline__ TV rowSub(const DMat<T>& A, const RowEnt<T>& e, int k0, int kEnd, int step, TV acc,
                                      TV scale, const TX* x) {
-    int k = k0;
-    const int n = (kEnd - k0) * step;  // entries left (step = +-1)
-    int left = n;
-    for (; left >= kUnroll; left -= kUnroll, k += kUnroll * step) {
+    const int n = (kEnd - k0) * step;  // entries (step = +-1)
+    for (int t0 = 0; t0 < n; t0 += kUnroll) {
         int c[kUnroll];
         T v[kUnroll];
         TX xv[kUnroll];
 #pragma unroll
         for (int j = 0; j < kUnroll; ++j) {
-            const long long p = e.at(k + j * step);
+            // positions past the row's end re-read its batch's first entry
+            // (cached) and are not added: no branch between the loads
+            const long long p = e.at(k0 + step * (t0 + j < n ? t0 + j : t0));
             c[j] = A.col[p];
             v[j] = A.val[p];
         }
 #pragma unroll
         for (int j = 0; j < kUnroll; ++j) xv[j] = x[c[j]];
 #pragma unroll
-        for (int j = 0; j < kUnroll; ++j) {
-            if (Scaled)
-                acc = acc - (scale * (TV)v[j]) * (TV)xv[j];
-            else
-                acc = acc - (TV)v[j] * (TV)xv[j];
-        }
-    }
-    for (; left > 0; --left, k += step) {
-        const long long p = e.at(k);
-        if (Scaled)
-            acc = acc - (scale * (TV)A.val[p]) * (TV)x[A.col[p]];
-        else
-            acc = acc - (TV)A.val[p] * (TV)x[A.col[p]];
+        for (int j = 0; j < kUnroll; ++j)
+            if (t0 + j < n) {
+                if (Scaled)
+                    acc = acc - (scale * (TV)v[j]) * (TV)xv[j];
+                else
+                    acc = acc - (TV)v[j] * (TV)xv[j];
+            }
     }
     return acc;
 }

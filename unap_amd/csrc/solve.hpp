@@ -67,23 +67,32 @@ struct LduObj {
     int n() const { return host.n; }
 };
 
-// Levels of a schedule whose rows are few enough to run in one workgroup:
-// the first level index from which every level holds at most kTailRows rows.
-constexpr int kTailRows = 2048;
-inline int tailStart(const std::vector<int>& start) {
+// Levels of a schedule small enough to run in one workgroup: the first level
+// index from which every level holds at most maxRows rows.  The workgroup
+// sits on one CU, whose gather rate bounds a level at about 1 us per 1000
+// entries (MI355X, coarse AMG levels), while a level of its own launch costs
+// about 5 us over the whole chip: a level goes to the tail when its rows hold
+// at most kTailEntries sweep entries (4096 measured best of 2048 / 4096 /
+// 8192 on 128^3 AMG-GS and DILU solves).
+constexpr int kTailEntries = 4096;
+inline int tailRows(double entriesPerRow) {
+    const double r = kTailEntries / std::max(1.0, entriesPerRow);
+    return (int)std::min(2048.0, std::max(64.0, r));
+}
+inline int tailStart(const std::vector<int>& start, int maxRows) {
     const int nl = (int)start.size() - 1;
     int k = nl;
-    while (k > 0 && start[k] - start[k - 1] <= kTailRows) --k;
+    while (k > 0 && start[k] - start[k - 1] <= maxRows) --k;
     return k;
 }
 
 // One pass over a level schedule: wide levels one kernel each, the narrow
 // tail in one workgroup.
 template <typename T, typename TV, int MODE>
-void runSchedule(SweepArgs<T, TV> a, const std::vector<int>& levStartH, Skip sk) {
+void runSchedule(SweepArgs<T, TV> a, const std::vector<int>& levStartH, int maxRows, Skip sk) {
     Ctx& c = Ctx::get();
     const int nl = (int)levStartH.size() - 1;
-    const int tail = tailStart(levStartH);
+    const int tail = tailStart(levStartH, maxRows);
     for (int k = 0; k < tail; ++k) {
         const int p0 = levStartH[k], p1 = levStartH[k + 1];
         if (p1 <= p0) continue;
@@ -103,12 +112,13 @@ void gsSmooth(DevMatrix<T>& M, T* x, const T* b, int nSweeps, Skip sk) {
     if (!M.sweepable) throw Error(-3, "Gauss-Seidel needs faces sorted by owner with owner < neighbour");
     SweepArgs<T, T> a{M.view(), x, b, nullptr, M.fwdRows.p, M.fwdStart.get()};
     const int nl = (int)M.fwdStartH.size() - 1;
-    if (tailStart(M.fwdStartH) == 0) {
+    const int maxRows = tailRows(M.entriesPerRow);  // a GS row reads all its entries
+    if (tailStart(M.fwdStartH, maxRows) == 0) {
         kSweepTail<T, T, SWEEP_GS><<<1, 1024, 0, Ctx::get().stream>>>(a, 0, nl, nSweeps, sk);
         UH_LAUNCHED();
         return;
     }
-    for (int s = 0; s < nSweeps; ++s) runSchedule<T, T, SWEEP_GS>(a, M.fwdStartH, sk);
+    for (int s = 0; s < nSweeps; ++s) runSchedule<T, T, SWEEP_GS>(a, M.fwdStartH, maxRows, sk);
 }
 
 // DIC / DILU application w = M^-1 r: forward then backward schedule.
@@ -117,9 +127,10 @@ void luSweeps(DevMatrix<T>& M, bool dic, TV* w, const TV* r, const TV* rD, Skip 
     if (!M.sweepable) throw Error(-3, "DIC/DILU needs faces sorted by owner with owner < neighbour");
     const DMat<T> Av = dic ? M.viewU() : M.view();
     SweepArgs<T, TV> f{Av, w, r, rD, M.fwdRows.p, M.fwdStart.get()};
-    runSchedule<T, TV, SWEEP_FWD>(f, M.fwdStartH, sk);
+    const int maxRows = tailRows(0.5 * M.entriesPerRow);  // one side of each row
+    runSchedule<T, TV, SWEEP_FWD>(f, M.fwdStartH, maxRows, sk);
     SweepArgs<T, TV> bk{Av, w, nullptr, rD, M.bwdRows.get(), M.bwdStart.get()};
-    runSchedule<T, TV, SWEEP_BWD>(bk, M.bwdStartH, sk);
+    runSchedule<T, TV, SWEEP_BWD>(bk, M.bwdStartH, maxRows, sk);
 }
 
 // Work vectors of one level.
