@@ -28,8 +28,8 @@ namespace unap_hip {
 struct BatchSys {
     int n, nf, type, nFwd, nBwd;
     int cellOff;            // cells / rows / internal vectors / sweep positions
-    long long entOff;       // SELL entries
-    int sliceOff;           // slice starts (n_slices + 1 per system)
+    long long ellOff;       // ELL entries (n * ellW per system)
+    int ellW;               // ELL width: the system's longest row
     int fwdOff, bwdOff;     // level boundaries
     long long tabOff[2];    // sweep-table entries (forward, backward)
     int tabM[2];            // entry slots per position (forward, backward)
@@ -55,10 +55,9 @@ struct BatchArgs {
     const int* order;  // queue order (largest first)
     int nsys;
     unsigned* ticket;
-    const long long* sliceStart;
     const uint32_t* rowInfo;
-    const int* col;
-    const double* val;
+    const int* ellCol;
+    const double* ellVal;
     const double* diag;
     const int* fwdStart;
     const int* bwdStart;
@@ -109,8 +108,43 @@ __device__ __forceinline__ void bsum2(double& v0, double& v1, double* red) {
     v1 = c;
 }
 
+// The system matrix in ELL form: entry k of row i at k * n + i (rowAmul's
+// entry order), rows shorter than w padded.  Every entry address follows from
+// the row alone, so a row's loads start together instead of after its slice
+// start and row length arrive.
+struct EllMat {
+    const int* col;
+    const double* val;
+    const uint32_t* info;  // len | nLowerSide << 16
+    const double* diag;
+    int n, w;
+};
+constexpr int kEllBatch = 8;
+
+// rowAmul's arithmetic: acc = 0 + d x_i, then + v_k x[c_k] in entry order.
+__device__ __forceinline__ double ellAmul(const EllMat& A, int i, const double* x) {
+    const int len = (int)(A.info[i] & 0xffffu);
+    double acc = 0.0 + A.diag[i] * x[i];
+    for (int k0 = 0; k0 < A.w; k0 += kEllBatch) {
+        int c[kEllBatch];
+        double v[kEllBatch], xv[kEllBatch];
+#pragma unroll
+        for (int j = 0; j < kEllBatch; ++j) {
+            const size_t e = (size_t)min(k0 + j, A.w - 1) * A.n + i;  // past the width: re-read, not added
+            c[j] = A.col[e];
+            v[j] = A.val[e];
+        }
+#pragma unroll
+        for (int j = 0; j < kEllBatch; ++j) xv[j] = x[c[j]];
+#pragma unroll
+        for (int j = 0; j < kEllBatch; ++j)
+            if (k0 + j < len) acc = acc + v[j] * xv[j];
+    }
+    return acc;
+}
+
 struct SysView {
-    DMat<double> A;
+    EllMat A;
     SweepTab fwd, bwd;
     const int* fs;
     const int* bs;
@@ -185,6 +219,39 @@ __device__ __forceinline__ void batchPrecond(const SysView& s, const double* r, 
     batchSweep<false>(s.bwd, s.bs, s.nBwd, r, w);
 }
 
+// Vector loops over rows i = t, t + nt, ... (< n), kVecRows rows at a time:
+// the rows' loads first (rows past n re-read row n - 1 and are not used),
+// then their work in row order, the same order as one row at a time, so the
+// sums are unchanged.  Stores between rows otherwise keep each row's loads
+// from starting before the previous row is done.
+constexpr int kVecRows = 4;
+template <class V, class Load, class Use>
+__device__ __forceinline__ void vecRows(int t, int n, int nt, Load&& load, Use&& use) {
+    for (int i0 = t; i0 < n; i0 += kVecRows * nt) {
+        V v[kVecRows];
+#pragma unroll
+        for (int u = 0; u < kVecRows; ++u) v[u] = load(min(i0 + u * nt, n - 1));
+#pragma unroll
+        for (int u = 0; u < kVecRows; ++u)
+            if (i0 + u * nt < n) use(i0 + u * nt, v[u]);
+    }
+}
+struct V1 {
+    double a;
+};
+struct V2 {
+    double a, b;
+};
+struct V3 {
+    double a, b, c;
+};
+struct V4 {
+    double a, b, c, d;
+};
+struct V5 {
+    double a, b, c, d, e;
+};
+
 // Shadow-residual restart threshold of the FP32-preconditioned BiCGStab.
 constexpr double kBicgRestart = 1.0e-6;
 
@@ -203,7 +270,7 @@ __device__ void batchPCG(const BatchArgs& a, const SysView& s, int off, unap_hip
     float* w32 = a.w32 + off;
     double acc = 0.0;
     for (int i = t; i < n; i += nt) {
-        const double ri = b[i] - rowAmul<double, double>(s.A, i, x);
+        const double ri = b[i] - ellAmul(s.A, i, x);
         r[i] = ri;
         p[i] = 0.0;
         zo[i] = 0.0;
@@ -219,11 +286,11 @@ __device__ void batchPCG(const BatchArgs& a, const SysView& s, int off, unap_hip
         const double rhoOld = rho;
         batchPrecond(s, r, w32);
         double rz = 0.0, rzo = 0.0;
-        for (int i = t; i < n; i += nt) {
-            const double z = (double)w32[i];
-            rz += z * r[i];
-            rzo += zo[i] * r[i];
-        }
+        vecRows<V3>(t, n, nt, [&](int i) { return V3{(double)w32[i], r[i], zo[i]}; },
+                    [&](int, const V3& v) {
+                        rz += v.a * v.b;
+                        rzo += v.c * v.b;
+                    });
         bsum2(rz, rzo, red);
         rho = rz;
         if (!(fabs(rho) > kSmallD)) {
@@ -231,26 +298,27 @@ __device__ void batchPCG(const BatchArgs& a, const SysView& s, int off, unap_hip
             break;
         }
         const double beta = (rz - rzo) / rhoOld;
-        for (int i = t; i < n; i += nt) {
-            const double z = (double)w32[i];
-            p[i] = (it == 0) ? z : z + beta * p[i];
-            zo[i] = z;
-        }
+        vecRows<V2>(t, n, nt, [&](int i) { return V2{(double)w32[i], p[i]}; },
+                    [&](int i, const V2& v) {
+                        p[i] = (it == 0) ? v.a : v.a + beta * v.b;
+                        zo[i] = v.a;
+                    });
         __syncthreads();
         acc = 0.0;
         for (int i = t; i < n; i += nt) {
-            const double v = rowAmul<double, double>(s.A, i, p);
+            const double v = ellAmul(s.A, i, p);
             w[i] = v;
             acc += v * p[i];
         }
         const double alpha = rho / bsum(acc, red);
         acc = 0.0;
-        for (int i = t; i < n; i += nt) {
-            x[i] += alpha * p[i];
-            const double ri = r[i] - alpha * w[i];
-            r[i] = ri;
-            acc += ri * ri;
-        }
+        vecRows<V4>(t, n, nt, [&](int i) { return V4{x[i], p[i], r[i], w[i]}; },
+                    [&](int i, const V4& v) {
+                        x[i] = v.a + alpha * v.b;
+                        const double ri = v.c - alpha * v.d;
+                        r[i] = ri;
+                        acc += ri * ri;
+                    });
         prev = fin;
         fin = sqrt(bsum(acc, red)) + kSmallD;
         ++it;
@@ -286,7 +354,7 @@ __device__ void batchBiCG(const BatchArgs& a, const SysView& s, int off, unap_hi
     float* w32 = a.w32 + off;
     double acc = 0.0;
     for (int i = t; i < n; i += nt) {
-        const double ri = b[i] - rowAmul<double, double>(s.A, i, x);
+        const double ri = b[i] - ellAmul(s.A, i, x);
         r[i] = ri;
         r0[i] = ri;
         p[i] = 0.0;
@@ -302,7 +370,7 @@ __device__ void batchBiCG(const BatchArgs& a, const SysView& s, int off, unap_hi
     for (;;) {
         const double rhoOld = rho;
         acc = 0.0;
-        for (int i = t; i < n; i += nt) acc += r0[i] * r[i];
+        vecRows<V2>(t, n, nt, [&](int i) { return V2{r0[i], r[i]}; }, [&](int, const V2& v) { acc += v.a * v.b; });
         rho = bsum(acc, red);
         // The FP32 preconditioner is not an exactly linear operator, and the
         // shadow residual loses bi-orthogonality with it: iteration counts
@@ -324,24 +392,26 @@ __device__ void batchBiCG(const BatchArgs& a, const SysView& s, int off, unap_hi
             break;
         }
         const double beta = (rho / rhoOld) * (alpha / omega);
-        for (int i = t; i < n; i += nt) p[i] = (k == 0) ? r[i] : r[i] + beta * (p[i] - omega * Ay[i]);
+        vecRows<V3>(t, n, nt, [&](int i) { return V3{r[i], p[i], Ay[i]}; },
+                    [&](int i, const V3& v) { p[i] = (k == 0) ? v.a : v.a + beta * (v.b - omega * v.c); });
         __syncthreads();
         batchPrecond(s, p, w32);
-        for (int i = t; i < n; i += nt) y[i] = (double)w32[i];
+        vecRows<V1>(t, n, nt, [&](int i) { return V1{(double)w32[i]}; }, [&](int i, const V1& v) { y[i] = v.a; });
         __syncthreads();
         acc = 0.0;
         for (int i = t; i < n; i += nt) {
-            const double v = rowAmul<double, double>(s.A, i, y);
+            const double v = ellAmul(s.A, i, y);
             Ay[i] = v;
             acc += r0[i] * v;
         }
         alpha = rho / bsum(acc, red);
         acc = 0.0;
-        for (int i = t; i < n; i += nt) {
-            const double si = r[i] - alpha * Ay[i];
-            sv[i] = si;
-            acc += si * si;
-        }
+        vecRows<V2>(t, n, nt, [&](int i) { return V2{r[i], Ay[i]}; },
+                    [&](int i, const V2& v) {
+                        const double si = v.a - alpha * v.b;
+                        sv[i] = si;
+                        acc += si * si;
+                    });
         fin = sqrt(bsum(acc, red)) + kSmallD;
         if (bconv(fin, init, a.tol, a.relTol)) {
             for (int i = t; i < n; i += nt) x[i] += alpha * y[i];
@@ -350,11 +420,11 @@ __device__ void batchBiCG(const BatchArgs& a, const SysView& s, int off, unap_hi
             break;
         }
         batchPrecond(s, sv, w32);
-        for (int i = t; i < n; i += nt) z[i] = (double)w32[i];
+        vecRows<V1>(t, n, nt, [&](int i) { return V1{(double)w32[i]}; }, [&](int i, const V1& v) { z[i] = v.a; });
         __syncthreads();
         double tt = 0.0, ts = 0.0;
         for (int i = t; i < n; i += nt) {
-            const double v = rowAmul<double, double>(s.A, i, z);
+            const double v = ellAmul(s.A, i, z);
             t_[i] = v;
             tt += v * v;
             ts += v * sv[i];
@@ -362,12 +432,13 @@ __device__ void batchBiCG(const BatchArgs& a, const SysView& s, int off, unap_hi
         bsum2(tt, ts, red);
         omega = ts / tt;
         acc = 0.0;
-        for (int i = t; i < n; i += nt) {
-            x[i] += alpha * y[i] + omega * z[i];
-            const double ri = sv[i] - omega * t_[i];
-            r[i] = ri;
-            acc += ri * ri;
-        }
+        vecRows<V5>(t, n, nt, [&](int i) { return V5{y[i], z[i], sv[i], t_[i], x[i]}; },
+                    [&](int i, const V5& v) {
+                        x[i] = v.e + (alpha * v.a + omega * v.b);
+                        const double ri = v.c - omega * v.d;
+                        r[i] = ri;
+                        acc += ri * ri;
+                    });
         prev = fin;
         fin = sqrt(bsum(acc, red)) + kSmallD;
         ++it;
@@ -401,8 +472,7 @@ __global__ void __launch_bounds__(BS) kBatchSolve(BatchArgs a) {
         const BatchSys d = a.sys[a.order[q]];
         SysView s;
         s.n = d.n;
-        s.A = {d.n, a.sliceStart + d.sliceOff, a.rowInfo + d.cellOff, a.col + d.entOff, a.val + d.entOff,
-               a.diag + d.cellOff};
+        s.A = {a.ellCol + d.ellOff, a.ellVal + d.ellOff, a.rowInfo + d.cellOff, a.diag + d.cellOff, d.n, d.ellW};
         SweepTab* tabs[2] = {&s.fwd, &s.bwd};
         for (int dir = 0; dir < 2; ++dir)
             *tabs[dir] = {a.tabRc[dir] + d.cellOff, a.tabRd[dir] + d.cellOff, a.tabCol[dir] + d.tabOff[dir],
@@ -435,10 +505,9 @@ struct unap_hip_batch_s {
     DBuf<BatchSys> sys;
     DBuf<int> order;
     DBuf<unsigned> ticket;
-    DBuf<long long> sliceStart;
     DBuf<uint32_t> rowInfo;
-    DBuf<int> col, fwdStart, bwdStart, perm;
-    DBuf<double> val, diag, x, b, v;
+    DBuf<int> ellCol, fwdStart, bwdStart, perm;
+    DBuf<double> ellVal, diag, x, b, v;
     DBuf<float> w32;
     DBuf<uint32_t> tabRc[2];
     DBuf<float> tabRd[2], tabCoef[2];
@@ -473,10 +542,9 @@ void launchBatch(unap_hip_batch_s& B, double relTol, double tol, long long maxIt
     a.order = B.order.get();
     a.nsys = B.nsys;
     a.ticket = B.ticket.get();
-    a.sliceStart = B.sliceStart.get();
     a.rowInfo = B.rowInfo.get();
-    a.col = B.col.get();
-    a.val = B.val.get();
+    a.ellCol = B.ellCol.get();
+    a.ellVal = B.ellVal.get();
     a.diag = B.diag.get();
     for (int dir = 0; dir < 2; ++dir) {
         a.tabRc[dir] = B.tabRc[dir].get();
@@ -520,14 +588,12 @@ int unap_hip_batch_create(int64_t nsys, const int64_t* cellOff, const int64_t* f
         auto* B = new unap_hip_batch_s;
         std::unique_ptr<unap_hip_batch_s> guardB(B);
         B->nsys = (int)nsys;
-        std::vector<long long> sliceAll;
         std::vector<uint32_t> infoAll;
-        std::vector<int> colAll, fwdAll, bwdAll, permAll;
-        std::vector<double> valAll, diagAll;
+        std::vector<int> ellColAll, fwdAll, bwdAll, permAll;
+        std::vector<double> ellValAll, diagAll;
         std::vector<uint32_t> tRc[2];
         std::vector<float> tRd[2], tCoef[2];
         std::vector<int> tCol[2];
-        long long ent = 0;
         std::vector<double> work(nsys);
         for (int64_t s = 0; s < nsys; ++s) {
             const int64_t n = cellOff[s + 1] - cellOff[s], nf = faceOff[s + 1] - faceOff[s];
@@ -549,7 +615,7 @@ int unap_hip_batch_create(int64_t nsys, const int64_t* cellOff, const int64_t* f
             A.up.assign(upper + faceOff[s], upper + faceOff[s] + nf);
             A.symm = (lower == nullptr || lower == upper);
             if (!A.symm) A.lo.assign(lower + faceOff[s], lower + faceOff[s] + nf);
-            Layout L = buildLayout(A, 0);  // sweeps dominate: level-major rows
+            Layout L = buildLayout(A, 1);  // breadth-first rows: the Amul gathers stay local
             if (!L.sweepable)
                 throw Error(UNAP_HIP_ERR_FACE_ORDER, "batch system faces must be owner-sorted with owner < neighbour");
             BatchSys d;
@@ -559,8 +625,9 @@ int unap_hip_batch_create(int64_t nsys, const int64_t* cellOff, const int64_t* f
             d.nFwd = (int)L.fwdStart.size() - 1;
             d.nBwd = (int)L.bwdStart.size() - 1;
             d.cellOff = (int)cellOff[s];
-            d.entOff = ent;
-            d.sliceOff = (int)sliceAll.size();
+            d.ellOff = (long long)ellColAll.size();
+            d.ellW = 0;
+            for (int i = 0; i < (int)n; ++i) d.ellW = std::max(d.ellW, (int)(L.rowInfo[i] & 0xffffu));
             d.fwdOff = (int)fwdAll.size();
             d.bwdOff = (int)bwdAll.size();
             const bool dic = (solver[s] == UNAP_HIP_SOLVER_PCG);
@@ -614,19 +681,34 @@ int unap_hip_batch_create(int64_t nsys, const int64_t* cellOff, const int64_t* f
                 }
             }
             B->sysH.push_back(d);
-            sliceAll.insert(sliceAll.end(), L.sliceStart.begin(), L.sliceStart.end());
             infoAll.insert(infoAll.end(), L.rowInfo.begin(), L.rowInfo.end());
-            colAll.insert(colAll.end(), L.col.begin(), L.col.end());
             fwdAll.insert(fwdAll.end(), L.fwdStart.begin(), L.fwdStart.end());
             bwdAll.insert(bwdAll.end(), L.bwdStart.begin(), L.bwdStart.end());
             for (int i = 0; i < (int)n; ++i) permAll.push_back((int)cellOff[s] + L.perm[i]);
             std::vector<double> v;
             fillSellValues<double>(L, A, false, v);
-            valAll.insert(valAll.end(), v.begin(), v.end());
+            // ELL copy of the SELL rows: entry k of row i at k * n + i; rows
+            // shorter than the width point their padding at themselves
+            const size_t eo = ellColAll.size();
+            ellColAll.resize(eo + (size_t)d.ellW * n);
+            ellValAll.resize(eo + (size_t)d.ellW * n, 0.0);
+            for (int i = 0; i < (int)n; ++i) {
+                const int len = (int)(L.rowInfo[i] & 0xffffu);
+                const long long base = L.sliceStart[i >> kSliceShift] + (i & (kSliceRows - 1));
+                for (int k = 0; k < d.ellW; ++k) {
+                    const size_t at = eo + (size_t)k * n + i;
+                    if (k < len) {
+                        const long long e = base + ((long long)k << kSliceShift);
+                        ellColAll[at] = L.col[e];
+                        ellValAll[at] = v[e];
+                    } else {
+                        ellColAll[at] = i;
+                    }
+                }
+            }
             std::vector<double> dd;
             fillDiag<double>(L, A, dd);
             diagAll.insert(diagAll.end(), dd.begin(), dd.end());
-            ent += L.nnzPadded;
             work[s] = (double)(n + 2 * nf) * std::cbrt((double)n) * (dic ? 1.0 : 2.0);
         }
         B->total = cellOff[nsys];
@@ -636,9 +718,12 @@ int unap_hip_batch_create(int64_t nsys, const int64_t* cellOff, const int64_t* f
         B->sys.upload(B->sysH, c.stream);
         B->order.upload(ord, c.stream);
         B->ticket.alloc(1);
-        B->sliceStart.upload(sliceAll, c.stream);
         B->rowInfo.upload(infoAll, c.stream);
-        B->col.upload(colAll, c.stream);
+        if (ellColAll.empty()) {  // diagonal systems only: keep the pointers valid
+            ellColAll.push_back(0);
+            ellValAll.push_back(0.0);
+        }
+        B->ellCol.upload(ellColAll, c.stream);
         B->fwdStart.upload(fwdAll, c.stream);
         B->bwdStart.upload(bwdAll, c.stream);
         for (int dir = 0; dir < 2; ++dir) {
@@ -652,7 +737,7 @@ int unap_hip_batch_create(int64_t nsys, const int64_t* cellOff, const int64_t* f
             B->tabCoef[dir].upload(tCoef[dir], c.stream);
         }
         B->perm.upload(permAll, c.stream);
-        B->val.upload(valAll, c.stream);
+        B->ellVal.upload(ellValAll, c.stream);
         B->diag.upload(diagAll, c.stream);
         B->x.alloc(B->total);
         B->b.alloc(B->total);
