@@ -58,6 +58,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-systems section")
     ap.add_argument("--no-configs", action="store_true", help="skip the C2 / C3 FP64 drop-in solves")
     ap.add_argument("--config-size", type=int, default=128, help="edge length of the C2 / C3 systems")
+    ap.add_argument("--configs", default="c2,c3", help="which drop-in configurations to time (c2, c3)")
+    ap.add_argument("--configs-only", action="store_true",
+                    help="time only the drop-in configurations (for kernel traces of exactly those solves)")
     ap.add_argument("--batch-launches", type=int, default=20)
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--kernel-only", default=None,
@@ -77,6 +80,10 @@ def main():
 
     dev_name, cus = hip.device_info()
     log("device:", dev_name, cus, "CUs")
+    if args.configs_only:
+        print(json.dumps({"configs": config_legs(args.config_size, not args.no_cpu, args.configs.split(","))}),
+              flush=True)
+        return
     N = args.size
     if args.kernel_only:
         s = meshgen.config_system(4, N)
@@ -174,7 +181,7 @@ def main():
     # per-kernel rooflines on the finest level (HIP events on the library stream)
     kernels = {}
     for name, prec in [("cheby", 32), ("cheby", 64), ("amul", 32), ("amul", 64), ("residual", 64),
-                       ("axpy_norm", 64), ("dot", 64), ("copy", 64), ("copy", 32)]:
+                       ("axpy_norm", 64), ("dot", 64), ("gs", 64), ("dic", 64), ("copy", 64), ("copy", 32)]:
         ms, by = hip.profile_kernel(A, name, prec, reps=20)
         gbs = by / (ms * 1e-3) / 1e9
         kernels[f"{name}_fp{prec}"] = {"ms": round(ms, 4), "alg_bytes": by, "GBps": round(gbs, 1),
@@ -199,7 +206,7 @@ def main():
     del P32, P64, P64f, S32, S64, S64f, Sref, A, x, b
     configs = None
     if not args.no_configs:
-        configs = config_legs(args.config_size, not args.no_cpu)
+        configs = config_legs(args.config_size, not args.no_cpu, args.configs.split(","))
 
     batch = None
     if not args.no_batch:
@@ -234,7 +241,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def config_legs(N, with_cpu, reps=3):
+def config_legs(N, with_cpu, which=("c2", "c3"), reps=3):
     """The FP64 drop-in configurations at their own sizes: C2 (PCG + AMG with
     Gauss-Seidel smoothing, coarsest level <= 500 cells) and C3 (PBiCGStab +
     DILU), rel. tol 1e-8, each timed on the GPU (median of `reps` solves from
@@ -249,6 +256,8 @@ def config_legs(N, with_cpu, reps=3):
     }
     out = {}
     for key, (solver, pre, amg, mg, what) in legs.items():
+        if key not in which:
+            continue
         s = meshgen.config_system(int(key[1]), N)
         A = hip.LduMatrix.from_system(s)
         t0 = time.time()

@@ -43,3 +43,14 @@ def test_shapes_merge_and_levels(tmp_path):
     assert abs(float(cells[4]) - b / 1e9) < 1e-2
     lvl = {ln.split("|")[1].strip(): ln for ln in lines if ln.startswith("| ") and ln.split("|")[1].strip().isdigit()}
     assert "| 4.0 |" in lvl["0"] and "| 1.0 |" in lvl["1"], out
+
+
+def test_time_table_last_solve(tmp_path):
+    init = "void unap_hip::kStateInit(unap_hip::KState*)"
+    lvl = "void unap_hip::kSweepLevel<double, double, 0>(unap_hip::SweepArgs<double, double>, int, int, unap_hip::Skip)"
+    seq = [(init, 5), (lvl, 500000), (init, 5), (lvl, 2000), (lvl, 4000)]
+    out = kernel_report.time_table(str(_trace(tmp_path, seq)))
+    row = [ln for ln in out.splitlines() if "kSweepLevel" in ln]
+    assert len(row) == 1, out
+    cells = [c.strip() for c in row[0].strip("|").split("|")]
+    assert cells[1] == "2" and abs(float(cells[3]) - 3.0) < 1e-9  # two launches, mean 3 us

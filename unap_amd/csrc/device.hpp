@@ -130,6 +130,28 @@ void launchReduce(const Body& b, int n, const int* order, const Fin& fin, Skip s
     UH_LAUNCHED();
 }
 
+// The backward sweep's entries in backward-level order: position q (row
+// bwdRows[q]) holds that row's owner-side entries from the last down to the
+// first, the order the sweep consumes them, in SELL slices over positions.
+// A backward level is then a contiguous run of positions, so its entry loads
+// coalesce although its rows are scattered in the forward-level-major layout.
+template <typename T>
+__global__ void kBuildBwdCopy(DMat<T> A, const int* __restrict__ bwdRows, const long long* __restrict__ bSlice,
+                              int* __restrict__ bCol, T* __restrict__ bVal, int n) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int row = bwdRows[q];
+    const long long src = A.sliceStart[row >> kSliceShift] + (row & (kSliceRows - 1));
+    const uint32_t info = A.rowInfo[row];
+    const int len = (int)(info & 0xffffu), mid = (int)(info >> 16);
+    const long long dst = bSlice[q >> kSliceShift] + (q & (kSliceRows - 1));
+    for (int j = 0; j < len - mid; ++j) {
+        const long long from = src + ((long long)(len - 1 - j) << kSliceShift);
+        bCol[dst + ((long long)j << kSliceShift)] = A.col[from];
+        bVal[dst + ((long long)j << kSliceShift)] = A.val[from];
+    }
+}
+
 // Device copy of one matrix in precision T.
 template <typename T>
 struct DevMatrix {
@@ -145,10 +167,38 @@ struct DevMatrix {
     bool sweepable = false;
     int shape = 0;  // rowAmul load shape of the V-cycle kernels: kShapeBatch4 above 14 entries a row
     double entriesPerRow = 0;  // off-diagonal entries a row, on average
+    // backward-sweep copy (kBuildBwdCopy): structure from the layout, entries
+    // gathered on the device on first use and dropped when the values change
+    std::vector<long long> bwdSliceH;
+    std::vector<uint32_t> bwdInfoH;
+    DBuf<long long> bwdSlice;
+    DBuf<uint32_t> bwdInfo;
+    DBuf<int> bwdCol;
+    DBuf<T> bwdVal;
 
     DMat<T> view() const { return {n, sliceStart.get(), rowInfo.get(), col.get(), val.get(), diag.get()}; }
     DMat<T> viewU() const {
         return {n, sliceStart.get(), rowInfo.get(), col.get(), valU.p ? valU.get() : val.get(), diag.get()};
+    }
+    // The backward copy as a matrix over positions (row length = owner-side
+    // entries, no lower side); built on first call.  Owner-side coefficients
+    // are the upper ones in both val and valU.
+    DMat<T> bwdView(hipStream_t s) {
+        if (!bwdVal.p) {
+            if (!bwdSlice.p) {
+                bwdSlice.upload(bwdSliceH, s);
+                bwdInfo.upload(bwdInfoH, s);
+            }
+            const long long nnz = bwdSliceH.empty() ? 0 : bwdSliceH.back();
+            bwdCol.alloc(std::max<long long>(1, nnz));
+            bwdVal.alloc(std::max<long long>(1, nnz));
+            if (n > 0) {
+                kBuildBwdCopy<T><<<(n + 255) / 256, 256, 0, s>>>(view(), bwdRows.get(), bwdSlice.get(), bwdCol.get(),
+                                                                  bwdVal.get(), n);
+                UH_LAUNCHED();
+            }
+        }
+        return {n, bwdSlice.get(), bwdInfo.get(), bwdCol.get(), bwdVal.get(), nullptr};
     }
 
     void build(const HostLdu& A, const Layout& L, hipStream_t s) {
@@ -173,6 +223,20 @@ struct DevMatrix {
         if (!L.fwdRows.empty()) fwdRows.upload(L.fwdRows, s);
         bwdRows.upload(L.bwdRows, s);
         bwdStart.upload(L.bwdStart, s);
+        if (sweepable) {
+            const int ns = (n + kSliceRows - 1) / kSliceRows;
+            bwdSliceH.assign(ns + 1, 0);
+            bwdInfoH.assign(n, 0);
+            for (int q = 0; q < n; ++q) {
+                const uint32_t info = L.rowInfo[L.bwdRows[q]];
+                bwdInfoH[q] = (info & 0xffffu) - (info >> 16);
+            }
+            for (int sl = 0; sl < ns; ++sl) {
+                uint32_t w = 0;
+                for (int q = sl * kSliceRows; q < std::min(n, (sl + 1) * kSliceRows); ++q) w = std::max(w, bwdInfoH[q]);
+                bwdSliceH[sl + 1] = bwdSliceH[sl] + (long long)w * kSliceRows;
+            }
+        }
         UH_CHECK(hipStreamSynchronize(s));  // host staging vectors die here
     }
     void buildValU(const HostLdu& A, const Layout& L, hipStream_t s) {
@@ -192,6 +256,7 @@ struct DevMatrix {
         diag.upload(d, s);
         UH_CHECK(hipStreamSynchronize(s));
         valU.release();
+        bwdVal.release();  // rebuilt from the new values on the next backward sweep
         if (hadU) buildValU(A, L, s);
     }
 };

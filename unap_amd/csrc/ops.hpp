@@ -558,14 +558,23 @@ struct SweepArgs {
     const int* levStart;  // level boundaries (device)
 };
 
+// Row at schedule position q.  Backward sweeps read the backward copy (A is
+// DevMatrix::bwdView: position q holds row rows[q]'s owner-side entries,
+// last first), the arithmetic of bwdRow.
 template <typename T, typename TV, int MODE>
-__device__ __forceinline__ void sweepRow(const SweepArgs<T, TV>& a, int r) {
+__device__ __forceinline__ void sweepRow(const SweepArgs<T, TV>& a, int q) {
+    if (MODE == SWEEP_BWD) {
+        const int r = a.rows[q];
+        RowEnt<T> e(a.A, q);
+        const TV rd = a.rD[r];
+        a.x[r] = rowSub<true, T, TV, TV>(a.A, e, 0, e.len(), 1, a.x[r], rd, a.x);
+        return;
+    }
+    const int r = a.rows ? a.rows[q] : q;
     if (MODE == SWEEP_GS)
         gsRow<T>(a.A, r, a.x, a.b);
-    else if (MODE == SWEEP_FWD)
-        fwdRow<T, TV>(a.A, r, a.x, a.b, a.rD);
     else
-        bwdRow<T, TV>(a.A, r, a.x, a.rD);
+        fwdRow<T, TV>(a.A, r, a.x, a.b, a.rD);
 }
 
 // One level: positions [p0, p1) of the schedule.
@@ -574,8 +583,7 @@ __global__ void __launch_bounds__(kBlock) kSweepLevel(SweepArgs<T, TV> a, int p0
     if (sk.on()) return;
     const int q = p0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= p1) return;
-    const int r = a.rows ? a.rows[q] : q;
-    sweepRow<T, TV, MODE>(a, r);
+    sweepRow<T, TV, MODE>(a, q);
 }
 
 // Levels [l0, l1) in one workgroup (the narrow tail of a schedule, or a whole
@@ -587,10 +595,7 @@ __global__ void __launch_bounds__(1024) kSweepTail(SweepArgs<T, TV> a, int l0, i
     for (int s = 0; s < nSweeps; ++s) {
         for (int k = l0; k < l1; ++k) {
             const int p0 = a.levStart[k], p1 = a.levStart[k + 1];
-            for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
-                const int r = a.rows ? a.rows[q] : q;
-                sweepRow<T, TV, MODE>(a, r);
-            }
+            for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) sweepRow<T, TV, MODE>(a, q);
             __syncthreads();
         }
     }

@@ -129,7 +129,7 @@ void luSweeps(DevMatrix<T>& M, bool dic, TV* w, const TV* r, const TV* rD, Skip 
     SweepArgs<T, TV> f{Av, w, r, rD, M.fwdRows.p, M.fwdStart.get()};
     const int maxRows = tailRows(0.5 * M.entriesPerRow);  // one side of each row
     runSchedule<T, TV, SWEEP_FWD>(f, M.fwdStartH, maxRows, sk);
-    SweepArgs<T, TV> bk{Av, w, nullptr, rD, M.bwdRows.get(), M.bwdStart.get()};
+    SweepArgs<T, TV> bk{M.bwdView(Ctx::get().stream), w, nullptr, rD, M.bwdRows.get(), M.bwdStart.get()};
     runSchedule<T, TV, SWEEP_BWD>(bk, M.bwdStartH, maxRows, sk);
 }
 

@@ -5,6 +5,9 @@ kernel trace.
         python3 bench.py --steps 1 --warmup 1 --mixed-only > bench.json
     python -m unap_amd.kernel_report OUT/run_kernel_trace.csv bench.json
 
+With the trace alone (python -m unap_amd.kernel_report TRACE), the last
+solve's kernels with their launch counts and times only (any solver).
+
 The last solve in the trace starts at its kStateInit dispatch.  For every
 kernel name in it: launches, total and mean time, algorithmic bytes (each
 distinct array a launch reads or writes, counted once at its unpadded length,
@@ -153,7 +156,31 @@ def per_level(seg, lv):
     return out
 
 
+def time_table(trace_csv):
+    """Launches, total and mean time per kernel of the last solve in a trace."""
+    rows = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
+    starts = [i for i, r in enumerate(rows) if "kStateInit" in r["Kernel_Name"]]
+    if not starts:
+        raise SystemExit("no kStateInit dispatch in the trace")
+    seg = rows[starts[-1]:]
+    agg = collections.defaultdict(lambda: [0, 0])
+    for r in seg:
+        a = agg[normalize(r["Kernel_Name"])]
+        a[0] += 1
+        a[1] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    busy = sum(v[1] for v in agg.values())
+    t0, t1 = int(seg[0]["Start_Timestamp"]), int(seg[-1]["End_Timestamp"])
+    out = [f"solve span {(t1 - t0) / 1e6:.1f} ms, kernel busy {busy / 1e6:.1f} ms, {len(seg)} dispatches", "",
+           "| kernel | launches | total ms | mean us | % of busy |", "|---|---|---|---|---|"]
+    for name, (c, d) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        out.append(f"| {short(name)} | {c} | {d / 1e6:.2f} | {d / c / 1e3:.1f} | {100 * d / busy:.1f} |")
+    return "\n".join(out)
+
+
 if __name__ == "__main__":
-    if len(sys.argv) != 3:
+    if len(sys.argv) == 2:
+        print(time_table(sys.argv[1]))
+    elif len(sys.argv) == 3:
+        print(report(sys.argv[1], sys.argv[2]))
+    else:
         raise SystemExit(__doc__)
-    print(report(sys.argv[1], sys.argv[2]))
