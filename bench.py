@@ -58,7 +58,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-systems section")
     ap.add_argument("--no-configs", action="store_true", help="skip the C2 / C3 FP64 drop-in solves")
     ap.add_argument("--config-size", type=int, default=128, help="edge length of the C2 / C3 systems")
-    ap.add_argument("--configs", default="c2,c3", help="which drop-in configurations to time (c2, c3)")
+    ap.add_argument("--configs", default="c1,c2,c3", help="which drop-in configurations to time (c1, c2, c3)")
     ap.add_argument("--configs-only", action="store_true",
                     help="time only the drop-in configurations (for kernel traces of exactly those solves)")
     ap.add_argument("--batch-launches", type=int, default=20)
@@ -241,28 +241,30 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def config_legs(N, with_cpu, which=("c2", "c3"), reps=3):
-    """The FP64 drop-in configurations at their own sizes: C2 (PCG + AMG with
-    Gauss-Seidel smoothing, coarsest level <= 500 cells) and C3 (PBiCGStab +
-    DILU), rel. tol 1e-8, each timed on the GPU (median of `reps` solves from
-    x = 0, setup excluded) beside the reference's x86 solve of the same system
-    (one and three iterations timed, extrapolated to the GPU's count)."""
+def config_legs(N, with_cpu, which=("c1", "c2", "c3"), reps=3):
+    """The FP64 drop-in configurations: C1 (PCG + DIC, 32^3 constant Poisson,
+    rel. tol 1e-6), and at edge N C2 (PCG + AMG with Gauss-Seidel smoothing,
+    coarsest level <= 500 cells) and C3 (PBiCGStab + DILU), rel. tol 1e-8.
+    Each is timed on the GPU (median of `reps` solves from x = 0, setup
+    excluded) beside the reference's x86 solve of the same system (one and
+    three iterations timed, extrapolated to the GPU's count)."""
     from unap_amd import hip, meshgen
 
     legs = {
-        "c2": ("PCG", "AMG", {"smoother": 1, "n_cells_coarsest": 500, "n_vcycles": 1}, [1, 500, 1, 0, 2, 2],
+        "c1": (32, 1e-6, "PCG", "DIC", None, None, "PCG + DIC"),
+        "c2": (N, 1e-8, "PCG", "AMG", {"smoother": 1, "n_cells_coarsest": 500, "n_vcycles": 1}, [1, 500, 1, 0, 2, 2],
                "PCG + FP64 AMG, Gauss-Seidel V(0,2), coarsest <= 500 cells"),
-        "c3": ("PBiCGStab", "DILU", None, None, "PBiCGStab + DILU"),
+        "c3": (N, 1e-8, "PBiCGStab", "DILU", None, None, "PBiCGStab + DILU"),
     }
     out = {}
-    for key, (solver, pre, amg, mg, what) in legs.items():
+    for key, (size, tol, solver, pre, amg, mg, what) in legs.items():
         if key not in which:
             continue
-        s = meshgen.config_system(int(key[1]), N)
+        s = meshgen.config_system(int(key[1]), size)
         A = hip.LduMatrix.from_system(s)
         t0 = time.time()
         P = hip.Precond(A, pre, amg=hip.amg_params(**amg) if amg else None)
-        S = hip.Solver(solver, P, rel_tol=1e-8, max_iter=1000)
+        S = hip.Solver(solver, P, rel_tol=tol, max_iter=1000)
         b = hip.DeviceArray.from_numpy(s.b)
         x = hip.DeviceArray(s.n)
         _, perf = S.solve(A, b, x=x)  # warm-up (first-call estimates)
@@ -277,7 +279,8 @@ def config_legs(N, with_cpu, which=("c2", "c3"), reps=3):
             times.append(time.perf_counter() - t1)
         t = float(np.median(times))
         res = _ldu_residual(s, x.numpy())
-        leg = {"workload": f"{key.upper()}: {N}^3 {what}", "s": round(t, 5), "iterations": perf["nIterations"],
+        leg = {"workload": f"{key.upper()}: {size}^3 {what}, rel. tol {tol:g}", "s": round(t, 5),
+               "iterations": perf["nIterations"],
                "converged": perf["converged"], "true_rel_residual": res, "setup_and_first_solve_s": round(t_setup, 2)}
         log(f"{key}: {t * 1e3:.1f} ms, {perf['nIterations']} iterations, converged={perf['converged']}, "
             f"true rel. residual {res:.2e}")
@@ -288,15 +291,20 @@ def config_legs(N, with_cpu, which=("c2", "c3"), reps=3):
             if path is None:
                 path, kind = O.oracle(), "port"
             M = path.matrix(s)
-            _, _, _, t_one = O.timed_solve(path, M, solver, pre, s.b, 1, 1, 1e-8, mg=mg)
-            _, _, _, t_three = O.timed_solve(path, M, solver, pre, s.b, 1, 3, 1e-8, mg=mg)
-            per_it = (t_three - t_one) / 2
-            est = t_one + (perf["nIterations"] - 1) * per_it
-            leg["cpu_baseline"] = {"value": round(est, 2), "unit": "s", "cores": 1, "kind": kind,
-                                   "sample": f"solves of 1 and 3 iterations ({t_one:.2f}s, {t_three:.2f}s) after a "
-                                             f"cold one, extrapolated to {perf['nIterations']} iterations"}
+            if s.n <= 100000:  # small enough to solve whole on the CPU
+                cpu_perf, _, _, est = O.timed_solve(path, M, solver, pre, s.b, 1, 1000, tol, mg=mg)
+                sample = f"the whole solve ({cpu_perf['nIterations']} iterations) after a cold one-iteration solve"
+                log(f"{key} cpu ({kind}): {est * 1e3:.1f} ms, {cpu_perf['nIterations']} iterations")
+            else:
+                _, _, _, t_one = O.timed_solve(path, M, solver, pre, s.b, 1, 1, tol, mg=mg)
+                _, _, _, t_three = O.timed_solve(path, M, solver, pre, s.b, 1, 3, tol, mg=mg)
+                per_it = (t_three - t_one) / 2
+                est = t_one + (perf["nIterations"] - 1) * per_it
+                sample = (f"solves of 1 and 3 iterations ({t_one:.2f}s, {t_three:.2f}s) after a cold one, "
+                          f"extrapolated to {perf['nIterations']} iterations")
+                log(f"{key} cpu ({kind}): {per_it:.2f}s an iteration -> {est:.1f}s")
+            leg["cpu_baseline"] = {"value": round(est, 4), "unit": "s", "cores": 1, "kind": kind, "sample": sample}
             leg["speedup_vs_cpu"] = round(est / t, 1)
-            log(f"{key} cpu ({kind}): {per_it:.2f}s an iteration -> {est:.1f}s")
         out[key] = leg
         del S, P, A, b, x
     return out

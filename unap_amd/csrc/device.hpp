@@ -167,6 +167,7 @@ struct DevMatrix {
     bool sweepable = false;
     int shape = 0;  // rowAmul load shape of the V-cycle kernels: kShapeBatch4 above 14 entries a row
     double entriesPerRow = 0;  // off-diagonal entries a row, on average
+    int maxRowLen = 0;
     // backward-sweep copy (kBuildBwdCopy): structure from the layout, entries
     // gathered on the device on first use and dropped when the values change
     std::vector<long long> bwdSliceH;
@@ -206,6 +207,7 @@ struct DevMatrix {
         sweepable = L.sweepable;
         shape = (2.0 * L.nf > 14.0 * L.n) ? kShapeBatch4 : kShapeWide8;
         entriesPerRow = L.n > 0 ? 2.0 * L.nf / L.n : 0.0;
+        maxRowLen = L.maxRowLen;
         sliceStart.upload(L.sliceStart, s);
         rowInfo.upload(L.rowInfo, s);
         col.upload(L.col, s);

@@ -586,6 +586,57 @@ __global__ void __launch_bounds__(kBlock) kSweepLevel(SweepArgs<T, TV> a, int p0
     sweepRow<T, TV, MODE>(a, q);
 }
 
+// One Gauss-Seidel level for long rows (coarse AMG levels, ~10-30 entries):
+// LPR lanes a row.  Lane i of a row's group loads, gathers and multiplies
+// entries i, i + LPR, ... of a chunk of LPR * S together; the group's first
+// lane then subtracts the chunk's products in entry order from LDS, so the
+// arithmetic is gsRow's while the row's loads are in flight at once rather
+// than kUnroll at a time.  maxLen bounds the rows' entry counts (the chunk
+// loop must be uniform for its barriers).
+template <typename T, int LPR>
+__global__ void __launch_bounds__(256) kGsLevelLanes(SweepArgs<T, T> a, int p0, int p1, int maxLen, Skip sk) {
+    if (sk.on()) return;
+    constexpr int S = 32 / LPR;
+    __shared__ T buf[256 * S];
+    const int i = (int)threadIdx.x % LPR;
+    T* mine = buf + threadIdx.x * S;
+    const T* group = buf + (threadIdx.x - i) * S;
+    const int q = p0 + (int)(blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+    const bool active = q < p1;
+    int r = 0, cnt = 0;
+    long long base = 0;
+    T acc = (T)0;
+    if (active) {
+        r = a.rows ? a.rows[q] : q;
+        base = a.A.sliceStart[r >> kSliceShift] + (r & (kSliceRows - 1));
+        cnt = (int)(a.A.rowInfo[r] & 0xffffu);
+        acc = a.b[r];
+    }
+    for (int t0 = 0; t0 < maxLen; t0 += LPR * S) {
+        int c[S];
+        T v[S], xv[S];
+#pragma unroll
+        for (int m = 0; m < S; ++m) {
+            const int t = t0 + i + LPR * m;
+            const long long p = base + ((long long)(t < cnt ? t : 0) << kSliceShift);
+            c[m] = t < cnt ? a.A.col[p] : 0;
+            v[m] = t < cnt ? a.A.val[p] : (T)0;
+        }
+#pragma unroll
+        for (int m = 0; m < S; ++m) xv[m] = (t0 + i + LPR * m < cnt) ? a.x[c[m]] : (T)0;
+#pragma unroll
+        for (int m = 0; m < S; ++m) mine[m] = v[m] * xv[m];
+        __syncthreads();
+        if (i == 0 && active) {
+            // entry t0 + LPR * m + j is slot m of lane j of the group
+            const int nn = min(cnt - t0, LPR * S);
+            for (int t = 0; t < nn; ++t) acc = acc - group[(t % LPR) * S + t / LPR];
+        }
+        __syncthreads();
+    }
+    if (active && i == 0) a.x[r] = acc / a.A.diag[r];
+}
+
 // Levels [l0, l1) in one workgroup (the narrow tail of a schedule, or a whole
 // small matrix), nSweeps times (GS only; sweeps re-read b every time).
 template <typename T, typename TV, int MODE>
