@@ -219,6 +219,25 @@ def test_amg_pcg_parity(oracle, order, sm, nv):
     _check_solve(S.solve(A, s.b), ref, ("AMG-PCG", sm, nv), order)
 
 
+def test_amg_gs_wide_coarse_levels_reference_order(oracle):
+    """AMG-PCG with Gauss-Seidel at 48^3 (C2 coefficients), bitwise against the
+    oracle in reference order.  Large enough that the second and third coarse
+    levels (12-15 entries a row, levels of several hundred rows) run their GS
+    levels as separate launches with 16 lanes a row (kGsLevelLanes), and every
+    level's sweeps split between level launches and the one-workgroup tail;
+    the small systems of the other tests only reach the tail."""
+    hip.set_reduction_order(hip.REDUCE_REFERENCE)
+    try:
+        s = _sys(2, 48)
+        A = hip.LduMatrix.from_system(s)
+        P = hip.Precond(A, "AMG", amg=hip.amg_params(smoother=1, n_cells_coarsest=500, n_vcycles=1))
+        S = hip.Solver("PCG", P, rel_tol=1e-8, max_iter=300)
+        ref = oracle.matrix(s).solve("PCG", "AMG", s.b, rel_tol=1e-8, max_iter=300, mg=[1, 500, 1, 0, 2, 2])
+        _check_solve(S.solve(A, s.b), ref, "AMG-PCG GS 48^3", "reference")
+    finally:
+        hip.set_reduction_order(hip.REDUCE_TREE)
+
+
 @pytest.mark.parametrize("pre,mg", [("DIC", None), ("AMG", [1, 20, 1, 0, 2, 2])])
 def test_fcg_fp64_parity(oracle, order, pre, mg):
     # the flexible outer loop in FP64 against the oracle's FCG
