@@ -62,7 +62,10 @@ def test_level_schedule_and_order(c, N):
     assert info["bwd_levels"] == blev.max() + 1
 
 
-@pytest.mark.parametrize("c,N,nc", [(2, 10, 20), (3, 10, 20), (2, 13, 50), (4, 12, 30), (1, 9, 10)])
+# the last two are large enough (over 65536 faces) for the multi-threaded
+# passes of the coarse addressing
+@pytest.mark.parametrize("c,N,nc", [(2, 10, 20), (3, 10, 20), (2, 13, 50), (4, 12, 30), (1, 9, 10), (2, 48, 500),
+                                    (3, 40, 200)])
 def test_agglomeration_matches_oracle(oracle, c, N, nc):
     s = meshgen.config_system(c, N)
     A = hip.LduMatrix.from_system(s)

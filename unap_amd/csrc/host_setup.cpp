@@ -5,13 +5,36 @@
 #include <cstring>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <numeric>
 #include <stdexcept>
+#include <thread>
 
 namespace unap_hip {
 
 static const double kSmall = 1.0e-37;  // SMALL, src/base/unap.hpp:88
+
+namespace {
+// f(begin, end) over [0, n) split among up to 16 host threads: for loops
+// whose iterations write disjoint elements, so the result does not depend
+// on the split.  Small ranges run on the calling thread.
+template <class F>
+void parallelFor(long long n, F&& f) {
+    const int T = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    if (T == 1 || n < (1 << 16)) {
+        f(0LL, n);
+        return;
+    }
+    const long long chunk = (n + T - 1) / T;
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) {
+        const long long a = t * chunk, b = std::min(n, a + chunk);
+        if (a < b) th.emplace_back([&f, a, b] { f(a, b); });
+    }
+    for (auto& x : th) x.join();
+}
+}  // namespace
 
 bool HostLdu::ownerSorted() const {
     const int nf = nFaces();
@@ -97,15 +120,19 @@ Layout finishLayout(const HostLdu& A, const RowFaces& R, std::vector<int> perm, 
     L.sweepable = sweepable;
     L.perm = std::move(perm);
     L.iperm.assign(n, 0);
-    for (int i = 0; i < n; ++i) L.iperm[L.perm[i]] = i;
+    parallelFor(n, [&](long long a, long long b) {
+        for (long long i = a; i < b; ++i) L.iperm[L.perm[i]] = (int)i;
+    });
 
     // Sweep schedules over internal rows.  When the internal order is level
     // major the forward rows are contiguous and fwdRows stays empty.
     std::vector<int> li(n), bi(n);
-    for (int i = 0; i < n; ++i) {
-        li[i] = lev[L.perm[i]];
-        bi[i] = blev[L.perm[i]];
-    }
+    parallelFor(n, [&](long long a, long long b) {
+        for (long long i = a; i < b; ++i) {
+            li[i] = lev[L.perm[i]];
+            bi[i] = blev[L.perm[i]];
+        }
+    });
     std::vector<int> fr = groupByLevel(li, nlev, L.fwdStart);
     bool contiguous = true;
     for (int i = 0; i < n && contiguous; ++i) contiguous = (fr[i] == i);
@@ -118,41 +145,51 @@ Layout finishLayout(const HostLdu& A, const RowFaces& R, std::vector<int> perm, 
     L.nslices = (n + C - 1) / C;
     L.sliceStart.assign(L.nslices + 1, 0);
     L.rowInfo.assign(n, 0);
-    for (int s = 0; s < L.nslices; ++s) {
-        int mx = 0;
-        for (int r = C * s; r < std::min(n, C * s + C); ++r) {
-            const int c = L.perm[r];
-            const int len = R.rs[c + 1] - R.rs[c];
-            if (len > 0xffff) throw std::runtime_error("row longer than 65535 entries");
-            mx = std::max(mx, len);
-            L.rowInfo[r] = (uint32_t)len | ((uint32_t)R.nLowerSide[c] << 16);
+    std::vector<int> width(L.nslices, 0);
+    std::atomic<bool> tooLong{false};
+    parallelFor(L.nslices, [&](long long a, long long b) {
+        for (long long s = a; s < b; ++s) {
+            int mx = 0;
+            for (int r = C * (int)s; r < std::min(n, C * (int)s + C); ++r) {
+                const int c = L.perm[r];
+                const int len = R.rs[c + 1] - R.rs[c];
+                if (len > 0xffff) tooLong = true;
+                mx = std::max(mx, len);
+                L.rowInfo[r] = (uint32_t)len | ((uint32_t)R.nLowerSide[c] << 16);
+            }
+            width[s] = mx;
         }
-        L.maxRowLen = std::max(L.maxRowLen, mx);
-        L.sliceStart[s + 1] = L.sliceStart[s] + (long long)C * mx;
+    });
+    if (tooLong) throw std::runtime_error("row longer than 65535 entries");
+    for (int s = 0; s < L.nslices; ++s) {
+        L.maxRowLen = std::max(L.maxRowLen, width[s]);
+        L.sliceStart[s + 1] = L.sliceStart[s] + (long long)C * width[s];
     }
     L.nnzPadded = L.sliceStart[L.nslices];
     L.col.assign(L.nnzPadded, 0);
     L.src.assign(L.nnzPadded, -1);
-    for (int s = 0; s < L.nslices; ++s) {
-        const int mx = (int)((L.sliceStart[s + 1] - L.sliceStart[s]) / C);
-        for (int lane = 0; lane < C; ++lane) {
-            const int r = C * s + lane;
-            const int c = r < n ? L.perm[r] : -1;
-            const int len = r < n ? R.rs[c + 1] - R.rs[c] : 0;
-            for (int k = 0; k < mx; ++k) {
-                const long long p = L.sliceStart[s] + (long long)C * k + lane;
-                if (k < len) {
-                    const int e = R.ent[R.rs[c] + k];
-                    const int f = e >> 1;
-                    L.col[p] = L.iperm[(e & 1) ? A.u[f] : A.l[f]];
-                    L.src[p] = e;
-                } else {
-                    L.col[p] = r < n ? r : 0;
-                    L.src[p] = -1;
+    parallelFor(L.nslices, [&](long long sa, long long sb) {
+        for (int s = (int)sa; s < (int)sb; ++s) {
+            const int mx = width[s];
+            for (int lane = 0; lane < C; ++lane) {
+                const int r = C * s + lane;
+                const int c = r < n ? L.perm[r] : -1;
+                const int len = r < n ? R.rs[c + 1] - R.rs[c] : 0;
+                for (int k = 0; k < mx; ++k) {
+                    const long long p = L.sliceStart[s] + (long long)C * k + lane;
+                    if (k < len) {
+                        const int e = R.ent[R.rs[c] + k];
+                        const int f = e >> 1;
+                        L.col[p] = L.iperm[(e & 1) ? A.u[f] : A.l[f]];
+                        L.src[p] = e;
+                    } else {
+                        L.col[p] = r < n ? r : 0;
+                        L.src[p] = -1;
+                    }
                 }
             }
         }
-    }
+    });
     return L;
 }
 
@@ -194,8 +231,15 @@ void sortWindows(std::vector<int>& perm, const RowFaces& R, int b, int e, int si
     if (sigma <= 1) return;
     auto len = [&](int c) { return R.rs[c + 1] - R.rs[c]; };
     for (int s = b; s < e; s += sigma) {
+        // stable insertion sort by descending length (the window is short and
+        // its lengths mostly equal, so this is near linear)
         const int t = std::min(e, s + sigma);
-        std::stable_sort(perm.begin() + s, perm.begin() + t, [&](int x, int y) { return len(x) > len(y); });
+        for (int i = s + 1; i < t; ++i) {
+            const int v = perm[i], lv = len(v);
+            int j = i;
+            for (; j > s && len(perm[j - 1]) < lv; --j) perm[j] = perm[j - 1];
+            perm[j] = v;
+        }
     }
 }
 
@@ -263,12 +307,14 @@ template <typename T>
 void fillSellValues(const Layout& L, const HostLdu& A, bool useUpperBothSides, std::vector<T>& val) {
     val.assign(L.nnzPadded, T(0));
     const double* lw = useUpperBothSides ? A.up.data() : A.lower();
-    for (long long p = 0; p < L.nnzPadded; ++p) {
-        const int e = L.src[p];
-        if (e < 0) continue;
-        const int f = e >> 1;
-        val[p] = (T)((e & 1) ? A.up[f] : lw[f]);
-    }
+    parallelFor(L.nnzPadded, [&](long long a, long long b) {
+        for (long long p = a; p < b; ++p) {
+            const int e = L.src[p];
+            if (e < 0) continue;
+            const int f = e >> 1;
+            val[p] = (T)((e & 1) ? A.up[f] : lw[f]);
+        }
+    });
 }
 
 template <typename T>
@@ -354,46 +400,53 @@ std::vector<int> pairCells(const HostLdu& A, const std::vector<double>& w, int& 
 
 // Coarse faces from fine faces joining different clusters, numbered owner-major
 // with neighbours in first-appearance order (lduAgglomerationAddressing.cpp:57-178).
+// The fine faces are bucketed by coarse owner (a counting sort that keeps
+// ascending face order), then each owner's bucket is walked once: a coarse
+// neighbour met for the first time in this bucket gets the next coarse face
+// id, which is its owner-major, first-appearance position.
 void coarseAddressing(const HostLdu& F, CoarseLevel& C) {
     const int nf = F.nFaces();
     const int nc = C.nCoarse;
-    std::vector<std::vector<int>> byOwner(nc);  // provisional face ids per coarse owner
-    std::vector<int> provNei;
     C.faceRestrictMap.assign(nf, 0);
-    for (int f = 0; f < nf; ++f) {
-        const int a = C.restrictMap[F.u[f]], b = C.restrictMap[F.l[f]];
-        if (a == b) {
-            C.faceRestrictMap[f] = -(a + 1);
-            continue;
-        }
-        const int own = std::min(a, b), nei = std::max(a, b);
-        int hit = -1;
-        for (int k : byOwner[own])
-            if (provNei[k] == nei) {
-                hit = k;
-                break;
+    // each fine face's coarse owner and neighbour (-1: inside one cluster),
+    // looked up once, in parallel: the ordered passes below then read them
+    // sequentially
+    std::vector<int> ownOf(nf), neiOf(nf);
+    parallelFor(nf, [&](long long fa, long long fb) {
+        for (long long f = fa; f < fb; ++f) {
+            const int a = C.restrictMap[F.u[f]], b = C.restrictMap[F.l[f]];
+            if (a == b) {
+                C.faceRestrictMap[f] = -(a + 1);
+                ownOf[f] = -1;
+            } else {
+                ownOf[f] = std::min(a, b);
+                neiOf[f] = std::max(a, b);
             }
-        if (hit < 0) {
-            hit = (int)provNei.size();
-            byOwner[own].push_back(hit);
-            provNei.push_back(nei);
         }
-        C.faceRestrictMap[f] = hit;
-    }
-    const int ncf = (int)provNei.size();
-    std::vector<int> renum(ncf);
-    C.A.n = nc;
-    C.A.l.resize(ncf);
-    C.A.u.resize(ncf);
-    int k2 = 0;
-    for (int c = 0; c < nc; ++c)
-        for (int k : byOwner[c]) {
-            C.A.l[k2] = c;
-            C.A.u[k2] = provNei[k];
-            renum[k] = k2++;
-        }
+    });
+    std::vector<int> start(nc + 1, 0);
     for (int f = 0; f < nf; ++f)
-        if (C.faceRestrictMap[f] >= 0) C.faceRestrictMap[f] = renum[C.faceRestrictMap[f]];
+        if (ownOf[f] >= 0) start[ownOf[f] + 1]++;
+    for (int c = 0; c < nc; ++c) start[c + 1] += start[c];
+    std::vector<int> bucket(start[nc]), at(start.begin(), start.end() - 1);
+    for (int f = 0; f < nf; ++f)
+        if (ownOf[f] >= 0) bucket[at[ownOf[f]]++] = f;
+    std::vector<int> seenBy(nc, -1), slot(nc, 0);
+    C.A.n = nc;
+    C.A.l.clear();
+    C.A.u.clear();
+    for (int own = 0; own < nc; ++own)
+        for (int k = start[own]; k < start[own + 1]; ++k) {
+            const int f = bucket[k];
+            const int nei = neiOf[f];
+            if (seenBy[nei] != own) {
+                seenBy[nei] = own;
+                slot[nei] = (int)C.A.l.size();
+                C.A.l.push_back(own);
+                C.A.u.push_back(nei);
+            }
+            C.faceRestrictMap[f] = slot[nei];
+        }
 }
 
 // Galerkin coarse coefficients (lduAgglomerateMatrix.cpp:3-104).

@@ -227,20 +227,29 @@ struct SmootherObj {
 // ----------------------------------------------------- agglomeration levels
 struct AgglHost {
     std::vector<CoarseLevel> levels;
-    std::vector<Layout> layouts[3];  // per layout mode, built on first use
-    const std::vector<Layout>& layoutsFor(int order) {
-        std::vector<Layout>& v = layouts[layoutMode(order)];
-        if (v.size() != levels.size()) {
-            v.clear();
-            for (auto& L : levels) v.push_back(buildLayout(L.A, layoutMode(order)));
+    // device layouts of the coarse levels per layout mode, each built when a
+    // consumer first asks for it (a hierarchy that keeps every second
+    // agglomeration level never builds the others)
+    std::vector<Layout> layouts[3];
+    std::vector<char> built[3];
+    const Layout& layoutAt(int order, int k) {
+        const int m = layoutMode(order);
+        if (layouts[m].size() != levels.size()) {
+            layouts[m].assign(levels.size(), Layout());
+            built[m].assign(levels.size(), 0);
         }
-        return v;
+        if (!built[m][k]) {
+            layouts[m][k] = buildLayout(levels[k].A, m);
+            built[m][k] = 1;
+        }
+        return layouts[m][k];
     }
 };
 inline void buildAggl(const HostLdu& fine, const std::vector<double>& w, int maxLevels, int nCoarsest,
                       AgglHost& out) {
     out.levels = agglomerate(fine, w, maxLevels, nCoarsest);
     for (auto& v : out.layouts) v.clear();
+    for (auto& v : out.built) v.clear();
 }
 
 // -------------------------------------------------------------- hierarchy

@@ -104,7 +104,6 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     fine = &Fine<T>::get(A, order);
     ag = agg;
     prm = p;
-    const std::vector<Layout>& lays = ag->layoutsFor(order);
     // hierarchy levels: every pairSteps-th agglomeration level, ending at the
     // coarsest one
     sel.clear();
@@ -125,8 +124,8 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     mats.resize(L);
     work.resize(L);
     for (int l = 0; l < L; ++l) {
-        const Layout& fl = (l == 0) ? A.layout(order) : lays[sel[l - 1]];
-        const Layout& cl = lays[sel[l]];
+        const Layout& fl = (l == 0) ? A.layout(order) : ag->layoutAt(order, sel[l - 1]);
+        const Layout& cl = ag->layoutAt(order, sel[l]);
         // restriction map from the previous hierarchy level: the composition
         // of the agglomeration maps in between
         const int k0 = (l == 0) ? 0 : sel[l - 1] + 1;
@@ -160,7 +159,7 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     mgst.alloc(1);
 
     const HostLdu& CA = lev(L - 1).A;
-    const Layout& CL = lays[sel[L - 1]];
+    const Layout& CL = ag->layoutAt(order, sel[L - 1]);
     if (!fixed) {
         // reference path: PCG+DIC / PBiCGStab+DILU to relTol 1e-2 in one workgroup
         coarsest64.reset();
@@ -1100,9 +1099,8 @@ static void ensureTransfers(unap_hip_aggl G) {
     if (!G->cs.empty()) return;
     Ctx& c = Ctx::get();
     for (size_t l = 0; l < G->ag->levels.size(); ++l) {
-        const std::vector<Layout>& lays = G->ag->layoutsFor(ORDER_STREAM);
-        const Layout& fl = (l == 0) ? G->A->layout(ORDER_STREAM) : lays[l - 1];
-        Transfer tr = buildTransfer(fl, lays[l], G->ag->levels[l].restrictMap);
+        const Layout& fl = (l == 0) ? G->A->layout(ORDER_STREAM) : G->ag->layoutAt(ORDER_STREAM, (int)l - 1);
+        Transfer tr = buildTransfer(fl, G->ag->layoutAt(ORDER_STREAM, (int)l), G->ag->levels[l].restrictMap);
         G->cs.emplace_back();
         G->ch.emplace_back();
         G->f2c.emplace_back();
@@ -1119,9 +1117,8 @@ int unap_hip_aggl_restrict_field(unap_hip_aggl G, int64_t l, const double* ff, d
         need(ff && cf, "null vector");
         ensureTransfers(G);
         Ctx& c = Ctx::get();
-        const std::vector<Layout>& lays = G->ag->layoutsFor(ORDER_STREAM);
-        const Layout& fl = (l == 0) ? G->A->layout(ORDER_STREAM) : lays[l - 1];
-        const Layout& cl = lays[l];
+        const Layout& fl = (l == 0) ? G->A->layout(ORDER_STREAM) : G->ag->layoutAt(ORDER_STREAM, (int)l - 1);
+        const Layout& cl = G->ag->layoutAt(ORDER_STREAM, (int)l);
         const int nf = fl.n, nc = cl.n;
         DBuf<int> fperm, cperm;
         fperm.upload(fl.perm, c.stream);
@@ -1150,9 +1147,8 @@ int unap_hip_aggl_prolong_field(unap_hip_aggl G, int64_t l, const double* cf, do
         need(ff && cf, "null vector");
         ensureTransfers(G);
         Ctx& c = Ctx::get();
-        const std::vector<Layout>& lays = G->ag->layoutsFor(ORDER_STREAM);
-        const Layout& fl = (l == 0) ? G->A->layout(ORDER_STREAM) : lays[l - 1];
-        const Layout& cl = lays[l];
+        const Layout& fl = (l == 0) ? G->A->layout(ORDER_STREAM) : G->ag->layoutAt(ORDER_STREAM, (int)l - 1);
+        const Layout& cl = G->ag->layoutAt(ORDER_STREAM, (int)l);
         const int nf = fl.n, nc = cl.n;
         DBuf<int> fperm, cperm;
         fperm.upload(fl.perm, c.stream);
