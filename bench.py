@@ -73,7 +73,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
-              "described in DESIGN.md and not run here.", file=sys.stderr)
+              "documented with the library (sharding over point-to-point xGMI) and not run here.", file=sys.stderr)
         sys.exit(2)
 
     from unap_amd import hip, meshgen
