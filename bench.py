@@ -256,6 +256,33 @@ def config_legs(N, with_cpu, which=("c1", "c2", "c3"), reps=3):
                "PCG + FP64 AMG, Gauss-Seidel V(0,2), coarsest <= 500 cells"),
         "c3": (N, 1e-8, "PBiCGStab", "DILU", None, None, "PBiCGStab + DILU"),
     }
+    # the precision pattern on the same systems: the preconditioner in FP32
+    # inside an FP64 flexible CG / BiCGStab
+    mixed_legs = {
+        "c1": ("FCG", "DIC", {"precision": 32}, "FCG + FP32 DIC"),
+        "c2": ("FCG", "AMG", {"precision": 32, "smoother": 1, "n_cells_coarsest": 500, "n_vcycles": 1,
+                              "cycle": hip.CYCLE_FIXED},
+               "FCG + FP32 AMG (Gauss-Seidel, fixed cycle, dense FP64 coarsest)"),
+        "c3": ("PBiCGStab", "DILU", {"precision": 32}, "PBiCGStab + FP32 DILU"),
+    }
+    # the FP64 solution the errors are measured against
+    ref_legs = {
+        "c1": ("PCG", "DIC", None),
+        "c2": ("FCG", "AMG", {"smoother": 1, "n_cells_coarsest": 500, "n_vcycles": 1}),
+        "c3": ("PBiCGStab", "DILU", None),
+    }
+
+    def timed(S, A, b, x):
+        _, perf = S.solve(A, b, x=x)  # warm-up (first-call estimates)
+        times = []
+        for _ in range(reps):
+            x.zero()
+            hip.synchronize()
+            t1 = time.perf_counter()
+            _, perf = S.solve(A, b, x=x)
+            hip.synchronize()
+            times.append(time.perf_counter() - t1)
+        return float(np.median(times)), perf
     out = {}
     for key, (size, tol, solver, pre, amg, mg, what) in legs.items():
         if key not in which:
@@ -267,23 +294,34 @@ def config_legs(N, with_cpu, which=("c1", "c2", "c3"), reps=3):
         S = hip.Solver(solver, P, rel_tol=tol, max_iter=1000)
         b = hip.DeviceArray.from_numpy(s.b)
         x = hip.DeviceArray(s.n)
-        _, perf = S.solve(A, b, x=x)  # warm-up (first-call estimates)
-        t_setup = time.time() - t0
-        times = []
-        for _ in range(reps):
-            x.zero()
-            hip.synchronize()
-            t1 = time.perf_counter()
-            _, perf = S.solve(A, b, x=x)
-            hip.synchronize()
-            times.append(time.perf_counter() - t1)
-        t = float(np.median(times))
-        res = _ldu_residual(s, x.numpy())
+        t, perf = timed(S, A, b, x)
+        t_setup = time.time() - t0 - reps * t
+        x64 = x.numpy()
+        res = _ldu_residual(s, x64)
         leg = {"workload": f"{key.upper()}: {size}^3 {what}, rel. tol {tol:g}", "s": round(t, 5),
                "iterations": perf["nIterations"],
                "converged": perf["converged"], "true_rel_residual": res, "setup_and_first_solve_s": round(t_setup, 2)}
         log(f"{key}: {t * 1e3:.1f} ms, {perf['nIterations']} iterations, converged={perf['converged']}, "
             f"true rel. residual {res:.2e}")
+        # FP32 preconditioner, and both solutions against an FP64 solve to 1e-12
+        msolver, mpre, mamg, mwhat = mixed_legs[key]
+        Pm = hip.Precond(A, mpre, amg=hip.amg_params(**mamg))
+        tm, pm = timed(hip.Solver(msolver, Pm, rel_tol=tol, max_iter=3000), A, b, x)
+        x32 = x.numpy()
+        rsolver, rpre, ramg = ref_legs[key]
+        Pr = hip.Precond(A, rpre, amg=hip.amg_params(**ramg) if ramg else None)
+        x.zero()
+        hip.Solver(rsolver, Pr, rel_tol=1e-12, max_iter=5000).solve(A, b, x=x)
+        xr = x.numpy()
+        e32 = float(np.linalg.norm(x32 - xr) / np.linalg.norm(xr))
+        e64 = float(np.linalg.norm(x64 - xr) / np.linalg.norm(xr))
+        leg["err_vs_fp64_1e-12"] = e64
+        leg["mixed"] = {"solver": mwhat, "s": round(tm, 5), "iterations": pm["nIterations"],
+                        "converged": pm["converged"], "true_rel_residual": _ldu_residual(s, x32),
+                        "err_vs_fp64_1e-12": e32, "speedup_vs_fp64": round(t / tm, 3)}
+        log(f"{key} mixed ({mwhat}): {tm * 1e3:.1f} ms, {pm['nIterations']} iterations; error vs FP64 1e-12: "
+            f"mixed {e32:.2e}, fp64 {e64:.2e}")
+        del Pm, Pr
         if with_cpu:
             from oracle import oracle as O
             path = O.reference()

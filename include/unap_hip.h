@@ -127,6 +127,11 @@ typedef struct {
 #define UNAP_HIP_CYCLE_FIXED 1
 
 int unap_hip_amg_params_default(unap_hip_amg_params* p);
+/* amg: the AMG parameters for UNAP_HIP_PRECOND_AMG (NULL: defaults).  For
+   DIC and DILU only amg->precision is read (NULL: 64): 32 runs the sweeps in
+   FP32 (FP32 coefficients and rD, the factorisation's rD computed in FP64 and
+   rounded) as the preconditioner of an FP64 UNAP_HIP_SOLVER_FCG or
+   UNAP_HIP_SOLVER_PBICGSTAB; no counterpart in the reference. */
 int unap_hip_precond_create(unap_hip_ldu A, int type, const unap_hip_amg_params* amg,
                             const double* face_weights, unap_hip_precond* out);
 int unap_hip_precond_destroy(unap_hip_precond P);

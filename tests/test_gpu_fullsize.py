@@ -75,3 +75,26 @@ def test_c4_mixed_fcg_256():
     assert p["converged"], p
     assert _true_rel_residual(s, x) <= 2e-8
     assert p["nIterations"] < 150
+
+
+def test_c2_fp32_amg_gs_fcg_128():
+    # the precision pattern on C2: FCG around the FP32 AMG with Gauss-Seidel
+    # (fixed cycle, dense FP64 coarsest level)
+    s = meshgen.config_system(2, 128)
+    A = hip.LduMatrix.from_system(s)
+    P = hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, smoother=1, n_cells_coarsest=500, n_vcycles=1,
+                                                   cycle=hip.CYCLE_FIXED))
+    S = hip.Solver("FCG", P, rel_tol=1e-8, max_iter=1000)
+    x, p, h = _solve_twice(S, A, s)
+    assert p["converged"], p
+    assert _true_rel_residual(s, x) <= 2e-8
+
+
+def test_c3_bicgstab_dilu32_128():
+    # the precision pattern on C3: PBiCGStab in FP64 around FP32 DILU sweeps
+    s = meshgen.config_system(3, 128)
+    A = hip.LduMatrix.from_system(s)
+    S = hip.Solver("PBiCGStab", hip.Precond(A, "DILU", precision=32), rel_tol=1e-8, max_iter=2000)
+    x, p, h = _solve_twice(S, A, s)
+    assert p["converged"], p
+    assert _true_rel_residual(s, x) <= 2e-8

@@ -324,7 +324,13 @@ struct PrecondObj {
     std::vector<double> rDext;       // external order (rD())
     std::unique_ptr<Hierarchy<double>> amg64;
     std::unique_ptr<Hierarchy<float>> amg32;
-    int precision() const { return amg32 ? 32 : 64; }
+    bool lu32 = false;   // DIC / DILU with FP32 sweeps
+    DBuf<float> rD32;    // their rD, internal order
+    int precision() const { return (amg32 || lu32) ? 32 : 64; }
+    // w = M^-1 r with FP32 sweeps (lu32), on FP32 vectors
+    void applyLu32(const float* r, float* w, Skip sk) {
+        luSweeps<float, float>(A->dev32(order), type == UNAP_HIP_PRECOND_DIC, w, r, rD32.get(), sk);
+    }
 };
 
 // precondition(w, r) in FP64 (types 0-3 and the FP64 hierarchy).

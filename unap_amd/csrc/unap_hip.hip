@@ -430,6 +430,7 @@ void applyPrecond64(PrecondObj& P, const double* r, double* w, Skip sk) {
             break;
         case UNAP_HIP_PRECOND_DIC:
         case UNAP_HIP_PRECOND_DILU:
+            if (P.lu32) throw Error(-4, "FP32 DIC/DILU preconditioners need the FCG or PBiCGStab solver");
             luSweeps<double, double>(P.A->dev64(P.order), P.type == UNAP_HIP_PRECOND_DIC, w, r, P.rD.get(), sk);
             break;
         case UNAP_HIP_PRECOND_AMG:
@@ -526,10 +527,16 @@ struct Driver {
     }
 
     void precondInto(const double* in, double* out, Skip sk) {
-        if (!S.P)
+        if (!S.P) {
             launchRows(BCopy<double, double>{in, out}, n, sk);
-        else
+        } else if (S.P->lu32) {
+            // FP32 sweeps between an FP32 copy of the input and the output
+            launchRows(BCopy<double, float>{in, B.r32.get()}, n, sk);
+            S.P->applyLu32(B.r32.get(), B.z32[0].get(), sk);
+            launchRows(BCopy<float, double>{B.z32[0].get(), out}, n, sk);
+        } else {
             applyPrecond64(*S.P, in, out, sk);
+        }
     }
 
     void bicg() {
@@ -578,7 +585,12 @@ struct Driver {
     }
 
     void fcg() {
-        if (S.P && S.P->amg32) {
+        if (S.P && S.P->lu32) {
+            PrecondObj* P = S.P;
+            float* r32 = B.r32.get();
+            fcgWith<float>(B.z32[0].get(), B.z32[1].get(), r32,
+                           [P, r32](const double*, float* z, Skip sk) { P->applyLu32(r32, z, sk); });
+        } else if (S.P && S.P->amg32) {
             Hierarchy<float>* H = S.P->amg32.get();
             float* r32 = B.r32.get();
             fcgWith<float>(B.z32[0].get(), B.z32[1].get(), r32,
@@ -877,7 +889,15 @@ int unap_hip_precond_create(unap_hip_ldu A, int type, const unap_hip_amg_params*
             std::vector<double> rin(H.n);
             for (int i = 0; i < H.n; ++i) rin[i] = o.rDext[L.perm[i]];
             o.rD.upload(rin, c.stream);
-            if (type == UNAP_HIP_PRECOND_DIC) A->o.dev64(o.order).buildValU(H, L, c.stream);
+            o.lu32 = (type != UNAP_HIP_PRECOND_DIAG) && amg != nullptr && amg->precision == 32;
+            need(amg == nullptr || amg->precision == 32 || amg->precision == 64, "precision must be 32 or 64");
+            if (o.lu32) {
+                std::vector<float> r32(rin.begin(), rin.end());
+                o.rD32.upload(r32, c.stream);
+                if (type == UNAP_HIP_PRECOND_DIC) A->o.dev32(o.order).buildValU(H, L, c.stream);
+            } else if (type == UNAP_HIP_PRECOND_DIC) {
+                A->o.dev64(o.order).buildValU(H, L, c.stream);
+            }
             c.sync();
         } else if (type == UNAP_HIP_PRECOND_AMG) {
             AmgParams prm = toParams(amg);
@@ -910,6 +930,11 @@ int unap_hip_precond_apply(unap_hip_precond P, const double* r, double* w, int w
             DBuf<float> w32(M.n);
             o.amg32->initSmoothers();
             o.amg32->precondition(ri.get(), w32.get(), Skip{nullptr, nullptr});
+            launchRows(BCopy<float, double>{w32.get(), wi.get()}, M.n);
+        } else if (o.lu32) {
+            DBuf<float> r32(M.n), w32(M.n);
+            launchRows(BCopy<double, float>{ri.get(), r32.get()}, M.n);
+            o.applyLu32(r32.get(), w32.get(), Skip{nullptr, nullptr});
             launchRows(BCopy<float, double>{w32.get(), wi.get()}, M.n);
         } else {
             if (o.amg64) o.amg64->initSmoothers();
@@ -1329,6 +1354,8 @@ int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const do
         LduObj& L = A->o;
         need(!o.P || o.P->A == &L, "preconditioner was built for another matrix");
         need(!(o.P && o.P->amg32 && o.type != UNAP_HIP_SOLVER_FCG), "the FP32 AMG preconditioner needs FCG");
+        need(!(o.P && o.P->lu32 && o.type != UNAP_HIP_SOLVER_FCG && o.type != UNAP_HIP_SOLVER_PBICGSTAB),
+             "FP32 DIC/DILU preconditioners need FCG or PBiCGStab");
         need(o.type != UNAP_HIP_SOLVER_MG || o.mgSet, "MG solver needs unap_hip_solver_set_mg first");
         Ctx& c = Ctx::get();
         if (where == UNAP_HIP_DEVICE) UH_CHECK(hipDeviceSynchronize());
@@ -1337,7 +1364,7 @@ int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const do
         else if (o.type == UNAP_HIP_SOLVER_MG && o.mg.smoother == UNAP_HIP_SMOOTHER_GS) order = ORDER_SWEEP;
         DevMatrix<double>& M = L.dev64(order);
         const long long histCap = o.maxIter + 2;
-        S->bufs.ensure(M.n, histCap, o.P && o.P->amg32);
+        S->bufs.ensure(M.n, histCap, o.P && (o.P->amg32 || o.P->lu32));
         if (o.type == UNAP_HIP_SOLVER_MG && (!o.mgH || o.mgFor != &L)) {
             auto ag = agglFor(L, o.mgWeightsPtr, true, o.mg.maxLevels, o.mg.nCoarsest);
             o.mgH.reset(new Hierarchy<double>);

@@ -294,8 +294,11 @@ def profile_kernel(A, kernel, precision=64, reps=20):
 
 
 class Precond:
-    def __init__(self, A, kind, amg=None, weights=None):
+    def __init__(self, A, kind, amg=None, weights=None, precision=None):
+        """precision=32 with DIC / DILU: FP32 sweeps (for FCG and PBiCGStab)."""
         self.A = A
+        if precision is not None and amg is None:
+            amg = amg_params(precision=precision)
         h = _vp()
         w = None if weights is None else np.ascontiguousarray(weights, np.float64)
         _check(lib().unap_hip_precond_create(A.h, PRECOND[kind], None if amg is None else ctypes.byref(amg),
