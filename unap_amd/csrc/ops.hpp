@@ -556,6 +556,7 @@ struct SweepArgs {
     const TV* rD;  // FWD/BWD
     const int* rows;      // backward: row list, null = contiguous
     const int* levStart;  // level boundaries (device)
+    const TV* rDpos = nullptr;  // backward: rD by schedule position (rD[rows[q]]), or null
 };
 
 // Row at schedule position q.  Backward sweeps read the backward copy (A is
@@ -566,7 +567,7 @@ __device__ __forceinline__ void sweepRow(const SweepArgs<T, TV>& a, int q) {
     if (MODE == SWEEP_BWD) {
         const int r = a.rows[q];
         RowEnt<T> e(a.A, q);
-        const TV rd = a.rD[r];
+        const TV rd = a.rDpos ? a.rDpos[q] : a.rD[r];
         a.x[r] = rowSub<true, T, TV, TV>(a.A, e, 0, e.len(), 1, a.x[r], rd, a.x);
         return;
     }

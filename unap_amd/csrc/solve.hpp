@@ -141,14 +141,16 @@ void gsSmooth(DevMatrix<T>& M, T* x, const T* b, int nSweeps, Skip sk) {
 }
 
 // DIC / DILU application w = M^-1 r: forward then backward schedule.
+// rDpos: rD in backward-schedule order (rD[bwdRows[q]]), read contiguously by
+// the backward sweep, or null (then rD is gathered by row).
 template <typename T, typename TV>
-void luSweeps(DevMatrix<T>& M, bool dic, TV* w, const TV* r, const TV* rD, Skip sk) {
+void luSweeps(DevMatrix<T>& M, bool dic, TV* w, const TV* r, const TV* rD, Skip sk, const TV* rDpos) {
     if (!M.sweepable) throw Error(-3, "DIC/DILU needs faces sorted by owner with owner < neighbour");
     const DMat<T> Av = dic ? M.viewU() : M.view();
     SweepArgs<T, TV> f{Av, w, r, rD, M.fwdRows.p, M.fwdStart.get()};
     const int maxRows = tailRows(0.5 * M.entriesPerRow);  // one side of each row
     runSchedule<T, TV, SWEEP_FWD>(f, M.fwdStartH, maxRows, sk);
-    SweepArgs<T, TV> bk{M.bwdView(Ctx::get().stream), w, nullptr, rD, M.bwdRows.get(), M.bwdStart.get()};
+    SweepArgs<T, TV> bk{M.bwdView(Ctx::get().stream), w, nullptr, rD, M.bwdRows.get(), M.bwdStart.get(), rDpos};
     runSchedule<T, TV, SWEEP_BWD>(bk, M.bwdStartH, maxRows, sk);
 }
 
@@ -326,10 +328,12 @@ struct PrecondObj {
     std::unique_ptr<Hierarchy<float>> amg32;
     bool lu32 = false;   // DIC / DILU with FP32 sweeps
     DBuf<float> rD32;    // their rD, internal order
+    DBuf<double> rDpos;  // DIC / DILU: rD in backward-schedule order
+    DBuf<float> rD32pos;
     int precision() const { return (amg32 || lu32) ? 32 : 64; }
     // w = M^-1 r with FP32 sweeps (lu32), on FP32 vectors
     void applyLu32(const float* r, float* w, Skip sk) {
-        luSweeps<float, float>(A->dev32(order), type == UNAP_HIP_PRECOND_DIC, w, r, rD32.get(), sk);
+        luSweeps<float, float>(A->dev32(order), type == UNAP_HIP_PRECOND_DIC, w, r, rD32.get(), sk, rD32pos.get());
     }
 };
 

@@ -431,7 +431,8 @@ void applyPrecond64(PrecondObj& P, const double* r, double* w, Skip sk) {
         case UNAP_HIP_PRECOND_DIC:
         case UNAP_HIP_PRECOND_DILU:
             if (P.lu32) throw Error(-4, "FP32 DIC/DILU preconditioners need the FCG or PBiCGStab solver");
-            luSweeps<double, double>(P.A->dev64(P.order), P.type == UNAP_HIP_PRECOND_DIC, w, r, P.rD.get(), sk);
+            luSweeps<double, double>(P.A->dev64(P.order), P.type == UNAP_HIP_PRECOND_DIC, w, r, P.rD.get(), sk,
+                                     P.rDpos.get());
             break;
         case UNAP_HIP_PRECOND_AMG:
             if (!P.amg64) throw Error(-4, "FP32 AMG preconditioner needs the FCG solver");
@@ -891,12 +892,20 @@ int unap_hip_precond_create(unap_hip_ldu A, int type, const unap_hip_amg_params*
             o.rD.upload(rin, c.stream);
             o.lu32 = (type != UNAP_HIP_PRECOND_DIAG) && amg != nullptr && amg->precision == 32;
             need(amg == nullptr || amg->precision == 32 || amg->precision == 64, "precision must be 32 or 64");
+            std::vector<double> rpos;
+            if (type != UNAP_HIP_PRECOND_DIAG) {
+                rpos.resize(H.n);
+                for (int q = 0; q < H.n; ++q) rpos[q] = rin[L.bwdRows[q]];
+            }
             if (o.lu32) {
                 std::vector<float> r32(rin.begin(), rin.end());
                 o.rD32.upload(r32, c.stream);
+                std::vector<float> r32pos(rpos.begin(), rpos.end());
+                o.rD32pos.upload(r32pos, c.stream);
                 if (type == UNAP_HIP_PRECOND_DIC) A->o.dev32(o.order).buildValU(H, L, c.stream);
-            } else if (type == UNAP_HIP_PRECOND_DIC) {
-                A->o.dev64(o.order).buildValU(H, L, c.stream);
+            } else if (type != UNAP_HIP_PRECOND_DIAG) {
+                o.rDpos.upload(rpos, c.stream);
+                if (type == UNAP_HIP_PRECOND_DIC) A->o.dev64(o.order).buildValU(H, L, c.stream);
             }
             c.sync();
         } else if (type == UNAP_HIP_PRECOND_AMG) {
@@ -1239,7 +1248,7 @@ static void profileKernel(LduObj& L, int kernel, long long reps, double* avgMs, 
                 gsSmooth<T>(M, x.get(), b.get(), 1, none);
                 break;
             case UNAP_HIP_KERNEL_DIC:
-                luSweeps<double, double>(M64, true, w64.get(), r64.get(), rD.get(), none);
+                luSweeps<double, double>(M64, true, w64.get(), r64.get(), rD.get(), none, nullptr);
                 break;
             case UNAP_HIP_KERNEL_RESIDUAL:
                 launchReduce<1>(BResidual<T>{M.view(), x.get(), b.get(), y.get(), nullptr}, n, M.iperm.get(),
