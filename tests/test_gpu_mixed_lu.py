@@ -15,7 +15,7 @@ from unap_amd import hip, meshgen
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("kind,c,N", [("DIC", 1, 12), ("DIC", 2, 12), ("DILU", 3, 12), ("DILU", 2, 10)])
+@pytest.mark.parametrize("kind,c,N", [("DIC", 1, 12), ("DIC", 2, 12), ("DIC", 3, 12), ("DILU", 3, 12), ("DILU", 2, 10)])
 def test_fp32_apply_matches_fp64(oracle, kind, c, N):
     s = meshgen.config_system(c, N)
     A = hip.LduMatrix.from_system(s)
