@@ -142,6 +142,14 @@ class preconditioner : public handle<unap_hip_precond, unap_hip_precond_destroy>
         check(unap_hip_precond_create(A.get(), type, amg, weights, &h_));
     }
 
+    // parameters that carry only a precision (read by DIC / DILU)
+    static const unap_hip_amg_params* withPrecision(int precision) {
+        static thread_local unap_hip_amg_params p;
+        check(unap_hip_amg_params_default(&p));
+        p.precision = precision;
+        return &p;
+    }
+
   public:
     template <class SF>
     void precondition(SF& w, const SF& r) const {
@@ -153,24 +161,28 @@ class preconditioner : public handle<unap_hip_precond, unap_hip_precond_destroy>
     }
 };
 
+inline unap_hip_amg_params defaultAMG() {
+    unap_hip_amg_params p;
+    check(unap_hip_amg_params_default(&p));
+    return p;
+}
+
 struct identityPrecond : preconditioner {
     explicit identityPrecond(const lduMatrix& A) : preconditioner(A, UNAP_HIP_PRECOND_NONE) {}
 };
 struct diagPrecond : preconditioner {
     explicit diagPrecond(const lduMatrix& A) : preconditioner(A, UNAP_HIP_PRECOND_DIAG) {}
 };
+// precision 32: FP32 sweeps, for the FCG and PBiCGStab solvers (no
+// counterpart in the reference); 64: the reference's arithmetic.
 struct DICPrecond : preconditioner {
-    explicit DICPrecond(const lduMatrix& A) : preconditioner(A, UNAP_HIP_PRECOND_DIC) {}
+    explicit DICPrecond(const lduMatrix& A, int precision = 64)
+        : preconditioner(A, UNAP_HIP_PRECOND_DIC, withPrecision(precision)) {}
 };
 struct DILUPrecond : preconditioner {
-    explicit DILUPrecond(const lduMatrix& A) : preconditioner(A, UNAP_HIP_PRECOND_DILU) {}
+    explicit DILUPrecond(const lduMatrix& A, int precision = 64)
+        : preconditioner(A, UNAP_HIP_PRECOND_DILU, withPrecision(precision)) {}
 };
-
-inline unap_hip_amg_params defaultAMG() {
-    unap_hip_amg_params p;
-    check(unap_hip_amg_params_default(&p));
-    return p;
-}
 
 // The agglomeration-AMG preconditioner (MGPreconditioner.hpp:11-49 defines it
 // as a stub; here precondition = nVcycles V-cycles of MGSolver from w = 0).

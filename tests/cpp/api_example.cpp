@@ -29,7 +29,8 @@ int main() {
         if (std::fabs(x[0] - 2.0) > 1e-12 || std::fabs(x[1] + 2.0) > 1e-12) return fail("ex3");
     }
     // a permuted 16^3 log-uniform Poisson system: PCG+DIC, PBiCGStab+DILU,
-    // AMG-PCG and the MG solver all reach relTol 1e-8
+    // AMG-PCG, the MG solver, and FCG / PBiCGStab around FP32 DIC / DILU all
+    // reach relTol 1e-8
     const int64_t N = 16, n = N * N * N, nf = unap_meshgen_nfaces(N, N, N);
     std::vector<int64_t> l(nf), u(nf);
     std::vector<double> up(nf), d(n), b(n);
@@ -47,13 +48,18 @@ int main() {
     prm.smoother = UNAP_HIP_SMOOTHER_CHEBY;
     prm.n_vcycles = 1;
     MGPrecond amg(A, prm);
+    DICPrecond dic32(A, 32);
+    DILUPrecond dilu32(A, 32);
     PCG s1(dic);
     PBiCGStab s2(dilu);
     PCG s3(amg);
     MGSolver s4(prm);
-    solver* all[] = {&s1, &s2, &s3, &s4};
-    const char* names[] = {"PCG+DIC", "PBiCGStab+DILU", "AMG-PCG", "MGSolver"};
-    for (int k = 0; k < 4; ++k) {
+    FCG s5(dic32);
+    PBiCGStab s6(dilu32);
+    solver* all[] = {&s1, &s2, &s3, &s4, &s5, &s6};
+    const char* names[] = {"PCG+DIC", "PBiCGStab+DILU", "AMG-PCG", "MGSolver", "FCG+DIC(FP32)",
+                           "PBiCGStab+DILU(FP32)"};
+    for (int k = 0; k < 6; ++k) {
         all[k]->SET_relTol(1e-8);
         all[k]->SET_maxIter(500);
         std::vector<double> x(n, 0.0);
