@@ -61,6 +61,9 @@ def main():
     ap.add_argument("--configs", default="c1,c2,c3", help="which drop-in configurations to time (c1, c2, c3)")
     ap.add_argument("--configs-only", action="store_true",
                     help="time only the drop-in configurations (for kernel traces of exactly those solves)")
+    ap.add_argument("--configs-fp64-only", action="store_true",
+                    help="with --configs-only: skip the FP32-preconditioner and FP64-reference solves, so the "
+                         "last solve in a kernel trace is a timed FP64 one")
     ap.add_argument("--batch-launches", type=int, default=20)
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--kernel-only", default=None,
@@ -81,8 +84,8 @@ def main():
     dev_name, cus = hip.device_info()
     log("device:", dev_name, cus, "CUs")
     if args.configs_only:
-        print(json.dumps({"configs": config_legs(args.config_size, not args.no_cpu, args.configs.split(","))}),
-              flush=True)
+        print(json.dumps({"configs": config_legs(args.config_size, not args.no_cpu, args.configs.split(","),
+                                                 with_mixed=not args.configs_fp64_only)}), flush=True)
         return
     N = args.size
     if args.kernel_only:
@@ -241,7 +244,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def config_legs(N, with_cpu, which=("c1", "c2", "c3"), reps=3):
+def config_legs(N, with_cpu, which=("c1", "c2", "c3"), reps=3, with_mixed=True):
     """The FP64 drop-in configurations: C1 (PCG + DIC, 32^3 constant Poisson,
     rel. tol 1e-6), and at edge N C2 (PCG + AMG with Gauss-Seidel smoothing,
     coarsest level <= 500 cells) and C3 (PBiCGStab + DILU), rel. tol 1e-8.
@@ -305,23 +308,26 @@ def config_legs(N, with_cpu, which=("c1", "c2", "c3"), reps=3):
             f"true rel. residual {res:.2e}")
         # FP32 preconditioner, and both solutions against an FP64 solve to 1e-12
         msolver, mpre, mamg, mwhat = mixed_legs[key]
-        Pm = hip.Precond(A, mpre, amg=hip.amg_params(**mamg))
-        tm, pm = timed(hip.Solver(msolver, Pm, rel_tol=tol, max_iter=3000), A, b, x)
-        x32 = x.numpy()
-        rsolver, rpre, ramg = ref_legs[key]
-        Pr = hip.Precond(A, rpre, amg=hip.amg_params(**ramg) if ramg else None)
-        x.zero()
-        hip.Solver(rsolver, Pr, rel_tol=1e-12, max_iter=5000).solve(A, b, x=x)
-        xr = x.numpy()
-        e32 = float(np.linalg.norm(x32 - xr) / np.linalg.norm(xr))
-        e64 = float(np.linalg.norm(x64 - xr) / np.linalg.norm(xr))
-        leg["err_vs_fp64_1e-12"] = e64
-        leg["mixed"] = {"solver": mwhat, "s": round(tm, 5), "iterations": pm["nIterations"],
-                        "converged": pm["converged"], "true_rel_residual": _ldu_residual(s, x32),
-                        "err_vs_fp64_1e-12": e32, "speedup_vs_fp64": round(t / tm, 3)}
-        log(f"{key} mixed ({mwhat}): {tm * 1e3:.1f} ms, {pm['nIterations']} iterations; error vs FP64 1e-12: "
-            f"mixed {e32:.2e}, fp64 {e64:.2e}")
-        del Pm, Pr
+        if not with_mixed:
+            msolver = None
+        if msolver is not None:
+            Pm = hip.Precond(A, mpre, amg=hip.amg_params(**mamg))
+            tm, pm = timed(hip.Solver(msolver, Pm, rel_tol=tol, max_iter=3000), A, b, x)
+            x32 = x.numpy()
+            rsolver, rpre, ramg = ref_legs[key]
+            Pr = hip.Precond(A, rpre, amg=hip.amg_params(**ramg) if ramg else None)
+            x.zero()
+            hip.Solver(rsolver, Pr, rel_tol=1e-12, max_iter=5000).solve(A, b, x=x)
+            xr = x.numpy()
+            e32 = float(np.linalg.norm(x32 - xr) / np.linalg.norm(xr))
+            e64 = float(np.linalg.norm(x64 - xr) / np.linalg.norm(xr))
+            leg["err_vs_fp64_1e-12"] = e64
+            leg["mixed"] = {"solver": mwhat, "s": round(tm, 5), "iterations": pm["nIterations"],
+                            "converged": pm["converged"], "true_rel_residual": _ldu_residual(s, x32),
+                            "err_vs_fp64_1e-12": e32, "speedup_vs_fp64": round(t / tm, 3)}
+            log(f"{key} mixed ({mwhat}): {tm * 1e3:.1f} ms, {pm['nIterations']} iterations; error vs FP64 1e-12: "
+                f"mixed {e32:.2e}, fp64 {e64:.2e}")
+            del Pm, Pr
         if with_cpu:
             from oracle import oracle as O
             path = O.reference()
