@@ -45,7 +45,8 @@ def test_bad_arguments_return_status():
         hip.amg_params(smoother=5) and hip.Precond(A, "AMG", amg=hip.amg_params(smoother=5))
 
 
-@pytest.mark.parametrize("c,N", [(1, 8), (2, 9), (3, 9), (4, 8)])
+# 48^3 (325k faces) takes the multi-threaded row-face pass
+@pytest.mark.parametrize("c,N", [(1, 8), (2, 9), (3, 9), (4, 8), (3, 48)])
 def test_level_schedule_and_order(c, N):
     s = meshgen.config_system(c, N)
     A = hip.LduMatrix.from_system(s)

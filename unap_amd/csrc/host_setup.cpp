@@ -57,44 +57,141 @@ RowFaces rowFaces(const HostLdu& A) {
     const int n = A.n, nf = A.nFaces();
     RowFaces R;
     R.rs.assign(n + 1, 0);
-    for (int f = 0; f < nf; ++f) {
-        R.rs[A.u[f] + 1]++;
-        R.rs[A.l[f] + 1]++;
-    }
-    for (int c = 0; c < n; ++c) R.rs[c + 1] += R.rs[c];
     R.ent.resize(2 * (size_t)nf);
     R.nLowerSide.assign(n, 0);
-    std::vector<int> fillp(R.rs.begin(), R.rs.end() - 1);
-    for (int f = 0; f < nf; ++f) {
-        R.ent[fillp[A.u[f]]++] = 2 * f;
-        R.ent[fillp[A.l[f]]++] = 2 * f + 1;
-        R.nLowerSide[A.u[f]]++;
+    if (nf < (1 << 16)) {
+        for (int f = 0; f < nf; ++f) {
+            R.rs[A.u[f] + 1]++;
+            R.rs[A.l[f] + 1]++;
+        }
+        for (int c = 0; c < n; ++c) R.rs[c + 1] += R.rs[c];
+        std::vector<int> fillp(R.rs.begin(), R.rs.end() - 1);
+        for (int f = 0; f < nf; ++f) {
+            R.ent[fillp[A.u[f]]++] = 2 * f;
+            R.ent[fillp[A.l[f]]++] = 2 * f + 1;
+            R.nLowerSide[A.u[f]]++;
+        }
+        return R;
     }
+    // Large matrices on host threads: counts by atomic increments, entries
+    // placed through atomic cursors, then each row's short list sorted by
+    // face id -- the same lists as the sequential pass above.
+    std::vector<std::atomic<int>> cnt(n + 1), low(n);
+    parallelFor(n + 1, [&](long long a, long long b) {
+        for (long long i = a; i < b; ++i) cnt[i].store(0, std::memory_order_relaxed);
+    });
+    parallelFor(n, [&](long long a, long long b) {
+        for (long long i = a; i < b; ++i) low[i].store(0, std::memory_order_relaxed);
+    });
+    parallelFor(nf, [&](long long a, long long b) {
+        for (long long f = a; f < b; ++f) {
+            cnt[A.u[f] + 1].fetch_add(1, std::memory_order_relaxed);
+            cnt[A.l[f] + 1].fetch_add(1, std::memory_order_relaxed);
+            low[A.u[f]].fetch_add(1, std::memory_order_relaxed);
+        }
+    });
+    for (int c = 0; c < n; ++c) R.rs[c + 1] = R.rs[c] + cnt[c + 1].load(std::memory_order_relaxed);
+    parallelFor(n, [&](long long a, long long b) {
+        for (long long c = a; c < b; ++c) {
+            R.nLowerSide[c] = low[c].load(std::memory_order_relaxed);
+            cnt[c].store(R.rs[c], std::memory_order_relaxed);  // cursor
+        }
+    });
+    parallelFor(nf, [&](long long a, long long b) {
+        for (long long f = a; f < b; ++f) {
+            R.ent[cnt[A.u[f]].fetch_add(1, std::memory_order_relaxed)] = (int)(2 * f);
+            R.ent[cnt[A.l[f]].fetch_add(1, std::memory_order_relaxed)] = (int)(2 * f + 1);
+        }
+    });
+    parallelFor(n, [&](long long a, long long b) {
+        for (long long c = a; c < b; ++c) std::sort(R.ent.begin() + R.rs[c], R.ent.begin() + R.rs[c + 1]);
+    });
     return R;
 }
 
 // Breadth-first order over the cell graph: recovers the spatial locality
-// that the caller's numbering may have scrambled.
+// that the caller's numbering may have scrambled.  Seeds in ascending cell
+// id; a cell's neighbours in its row-face order.  Wide frontiers are expanded
+// on host threads in the same order: every unvisited neighbour keeps only its
+// first discovery (smallest frontier position, then row-face index), and the
+// threads' lists are joined in frontier order.
 std::vector<int> bfsOrder(const HostLdu& A, const RowFaces& R) {
     const int n = A.n;
     std::vector<int> bfs;
     bfs.reserve(n);
     std::vector<char> seen(n, 0);
+    auto nbr = [&](int k) {
+        const int f = R.ent[k] >> 1;
+        return (R.ent[k] & 1) ? A.u[f] : A.l[f];
+    };
+    std::vector<std::atomic<unsigned long long>> claim;
+    const unsigned long long kFree = ~0ull;
+    constexpr int kWide = 1 << 14;  // frontier size worth the threads
+    const int T = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
     for (int s = 0; s < n; ++s) {
         if (seen[s]) continue;
         seen[s] = 1;
-        size_t head = bfs.size();
         bfs.push_back(s);
-        while (head < bfs.size()) {
-            const int c = bfs[head++];
-            for (int k = R.rs[c]; k < R.rs[c + 1]; ++k) {
-                const int f = R.ent[k] >> 1;
-                const int nb = (R.ent[k] & 1) ? A.u[f] : A.l[f];
-                if (!seen[nb]) {
-                    seen[nb] = 1;
-                    bfs.push_back(nb);
+        size_t fb = bfs.size() - 1, fe = bfs.size();
+        while (fb < fe) {
+            if (fe - fb < (size_t)kWide || T == 1) {
+                for (size_t p = fb; p < fe; ++p) {
+                    const int c = bfs[p];
+                    for (int k = R.rs[c]; k < R.rs[c + 1]; ++k) {
+                        const int nb = nbr(k);
+                        if (!seen[nb]) {
+                            seen[nb] = 1;
+                            bfs.push_back(nb);
+                        }
+                    }
                 }
+            } else {
+                if (claim.empty()) {
+                    claim = std::vector<std::atomic<unsigned long long>>(n);
+                    parallelFor(n, [&](long long a, long long b) {
+                        for (long long i = a; i < b; ++i) claim[i].store(kFree, std::memory_order_relaxed);
+                    });
+                }
+                const long long m = (long long)(fe - fb);
+                const long long chunk = (m + T - 1) / T;
+                auto key = [&](size_t p, int k) { return ((unsigned long long)p << 24) | (unsigned)(k - R.rs[bfs[p]]); };
+                std::vector<std::thread> th;
+                for (int t = 0; t < T; ++t)  // first discoveries
+                    th.emplace_back([&, t] {
+                        const size_t pa = fb + t * chunk, pb = std::min(fe, (size_t)(fb + (t + 1) * chunk));
+                        for (size_t p = pa; p < pb; ++p)
+                            for (int k = R.rs[bfs[p]]; k < R.rs[bfs[p] + 1]; ++k) {
+                                const int nb = nbr(k);
+                                if (seen[nb]) continue;
+                                const unsigned long long kk = key(p, k);
+                                unsigned long long cur = claim[nb].load(std::memory_order_relaxed);
+                                while (kk < cur && !claim[nb].compare_exchange_weak(cur, kk, std::memory_order_relaxed)) {
+                                }
+                            }
+                    });
+                for (auto& x : th) x.join();
+                th.clear();
+                std::vector<std::vector<int>> part(T);
+                for (int t = 0; t < T; ++t)  // each thread's discoveries, in order
+                    th.emplace_back([&, t] {
+                        const size_t pa = fb + t * chunk, pb = std::min(fe, (size_t)(fb + (t + 1) * chunk));
+                        for (size_t p = pa; p < pb; ++p)
+                            for (int k = R.rs[bfs[p]]; k < R.rs[bfs[p] + 1]; ++k) {
+                                const int nb = nbr(k);
+                                if (!seen[nb] && claim[nb].load(std::memory_order_relaxed) == key(p, k))
+                                    part[t].push_back(nb);
+                            }
+                    });
+                for (auto& x : th) x.join();
+                for (int t = 0; t < T; ++t)
+                    for (int v : part[t]) {
+                        seen[v] = 1;
+                        claim[v].store(kFree, std::memory_order_relaxed);
+                        bfs.push_back(v);
+                    }
             }
+            fb = fe;
+            fe = bfs.size();
         }
     }
     return bfs;
