@@ -473,6 +473,12 @@ struct BResidPlain {  // r = b - A x (eigenDiagPCG.cpp:64-76)
 };
 
 // ================================================================ sweeps
+// A row's first SELL position and its info word (length | lower-side count << 16).
+struct RowEntRaw {
+    long long base = 0;
+    uint32_t info = 0;
+};
+
 // Row helpers.  e(k) = position of entry k of row r.
 template <typename T>
 struct RowEnt {
@@ -482,6 +488,7 @@ struct RowEnt {
         base = A.sliceStart[r >> kSliceShift] + (r & (kSliceRows - 1));
         info = A.rowInfo[r];
     }
+    __device__ explicit RowEnt(const RowEntRaw& e) : base(e.base), info(e.info) {}
     __device__ int len() const { return (int)(info & 0xffffu); }
     __device__ int mid() const { return (int)(info >> 16); }
     __device__ long long at(int k) const { return base + ((long long)k << kSliceShift); }
@@ -638,18 +645,138 @@ __global__ void __launch_bounds__(256) kGsLevelLanes(SweepArgs<T, T> a, int p0, 
     if (active && i == 0) a.x[r] = acc / a.A.diag[r];
 }
 
+// A tail row with everything but its x gathers in registers: its row index,
+// initial value (GS: b; forward: rD r; backward: w), scale (rD), diagonal
+// (GS), and the columns and coefficients of its first kUnroll entries.  The
+// tail loads a level's rows before the barrier that ends the level before it,
+// so what a level waits for after the barrier is only the gathers of values
+// that level wrote.  The arithmetic is sweepRow's, entry for entry.
+template <typename T, typename TV>
+struct TailRow {
+    int r = -1;  // row written; -1: none
+    int n = 0;   // entries to subtract
+    RowEntRaw e;
+    TV acc, scale;
+    T diag;
+    int c[kUnroll];
+    T v[kUnroll];
+};
+
+// Row index, initial value, scale and diagonal of the row at schedule
+// position q (q < 0: none).
+template <typename T, typename TV, int MODE>
+__device__ __forceinline__ void tailLoadHead(const SweepArgs<T, TV>& a, int q, TailRow<T, TV>& t) {
+    t.r = -1;
+    t.n = 0;
+    if (q < 0) return;
+    const int r = (MODE == SWEEP_BWD || a.rows) ? a.rows[q] : q;
+    const int at = MODE == SWEEP_BWD ? q : r;  // the backward copy is by position
+    t.e.base = a.A.sliceStart[at >> kSliceShift] + (at & (kSliceRows - 1));
+    t.e.info = a.A.rowInfo[at];
+    t.r = r;
+    if (MODE == SWEEP_GS) {
+        t.n = (int)(t.e.info & 0xffffu);
+        t.acc = (TV)a.b[r];
+        t.scale = (TV)1;
+        t.diag = a.A.diag[r];
+    } else if (MODE == SWEEP_FWD) {
+        t.n = (int)(t.e.info >> 16);
+        const TV rd = a.rD[r];
+        t.acc = rd * a.b[r];
+        t.scale = rd;
+    } else {
+        t.n = (int)(t.e.info & 0xffffu);
+        t.scale = a.rDpos ? a.rDpos[q] : a.rD[r];
+        t.acc = a.x[r];
+    }
+}
+
+// Columns and coefficients of the row's first kUnroll entries (positions
+// past its end re-read the first entry and are not used).
+template <typename T, typename TV>
+__device__ __forceinline__ void tailLoadEntries(const DMat<T>& A, TailRow<T, TV>& t) {
+    if (t.n <= 0) return;
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+        const long long p = t.e.base + ((long long)(j < t.n ? j : 0) << kSliceShift);
+        t.c[j] = A.col[p];
+        t.v[j] = A.val[p];
+    }
+}
+
+template <typename T, typename TV>
+__device__ __forceinline__ void tailGather(const TV* x, const TailRow<T, TV>& t, TV (&xv)[kUnroll]) {
+    if (t.n <= 0) return;
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) xv[j] = x[t.c[j]];
+}
+
+template <typename T, typename TV, int MODE>
+__device__ __forceinline__ void tailFinish(const SweepArgs<T, TV>& a, const TailRow<T, TV>& t,
+                                           const TV (&xv)[kUnroll]) {
+    if (t.r < 0) return;
+    constexpr bool Scaled = MODE != SWEEP_GS;
+    TV acc = t.acc;
+    if (t.n > 0) {
+#pragma unroll
+        for (int j = 0; j < kUnroll; ++j)
+            if (j < t.n) {
+                if (Scaled)
+                    acc = acc - (t.scale * (TV)t.v[j]) * xv[j];
+                else
+                    acc = acc - (TV)t.v[j] * xv[j];
+            }
+        if (t.n > kUnroll) {
+            RowEnt<T> e(t.e);
+            acc = rowSub<Scaled, T, TV, TV>(a.A, e, kUnroll, t.n, 1, acc, t.scale, a.x);
+        }
+    }
+    if (MODE == SWEEP_GS) acc = acc / (TV)t.diag;
+    a.x[t.r] = acc;
+}
+
 // Levels [l0, l1) in one workgroup (the narrow tail of a schedule, or a whole
-// small matrix), nSweeps times (GS only; sweeps re-read b every time).
+// small matrix), nSweeps times (GS only; sweeps re-read b every time).  Each
+// thread loads its first row of the next level before it finishes its row of
+// the current one, so the loads that do not depend on the sweep's values
+// overlap the gathers that do; rows past the block size (levels wider than
+// the block) run as sweepRow.  Level bounds are uniform scalar loads.
 template <typename T, typename TV, int MODE>
 __global__ void __launch_bounds__(1024) kSweepTail(SweepArgs<T, TV> a, int l0, int l1, int nSweeps,
                                                    Skip sk) {
     if (sk.on()) return;
-    for (int s = 0; s < nSweeps; ++s) {
-        for (int k = l0; k < l1; ++k) {
-            const int p0 = a.levStart[k], p1 = a.levStart[k + 1];
-            for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) sweepRow<T, TV, MODE>(a, q);
-            __syncthreads();
+    const int nl = l1 - l0;
+    const int steps = nSweeps * nl;
+    if (steps <= 0) return;
+    const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+    auto lev = [&](int k) { return a.levStart[l0 + k]; };
+    int p0 = lev(0), p1 = lev(1);
+    TailRow<T, TV> cur, nxt;
+    tailLoadHead<T, TV, MODE>(a, p0 + tid < p1 ? p0 + tid : -1, cur);
+    tailLoadEntries<T, TV>(a.A, cur);
+    // the loop is entered with no load in flight, as after each step's barrier
+    __syncthreads();
+    int k = 0;  // level of this step, relative to l0
+    for (int s = 0; s < steps; ++s) {
+        // gathers of this level first, then the next level's row heads, so
+        // the wait for the gathers does not cover the heads; the next
+        // level's entries (which need the heads) after this level's stores
+        TV xv[kUnroll];
+        tailGather<T, TV>(a.x, cur, xv);
+        int n0 = 0, n1 = 0;
+        if (s + 1 < steps) {
+            k = (k + 1 == nl) ? 0 : k + 1;
+            n0 = lev(k);
+            n1 = lev(k + 1);
+            tailLoadHead<T, TV, MODE>(a, n0 + tid < n1 ? n0 + tid : -1, nxt);
         }
+        tailFinish<T, TV, MODE>(a, cur, xv);
+        for (int q = p0 + tid + nt; q < p1; q += nt) sweepRow<T, TV, MODE>(a, q);
+        if (s + 1 < steps) tailLoadEntries<T, TV>(a.A, nxt);
+        __syncthreads();
+        cur = nxt;
+        p0 = n0;
+        p1 = n1;
     }
 }
 
