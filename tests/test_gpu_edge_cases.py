@@ -10,7 +10,8 @@ level schedule, against the CPU oracle:
     several breadth-first seeds, rows with no entries between full ones).
 
 Bars: Amul, preconditioner application and Gauss-Seidel bit-exact; solvers in
-reference-order reductions bit-exact (iterations, residual history, solution).
+reference-order reductions bit-exact (iterations, residual history, solution);
+the batched solve (FP32 preconditioners) to its tolerance and the FP64 error.
 """
 import numpy as np
 import pytest
@@ -108,3 +109,30 @@ def test_edge_mg_reference_order(oracle, reference_order, case):
     assert p["nIterations"] == po["nIterations"], (case, p["nIterations"], po["nIterations"])
     assert np.array_equal(p["history"], po["history"]), case
     assert np.array_equal(x, xo), case
+
+
+def test_edge_batch(oracle):
+    # the edge systems inside one batched launch, beside ordinary ones: FP32
+    # DIC / DILU in FP64 flexible CG / BiCGStab, each system checked against
+    # its own residual and the oracle's FP64 solution
+    systems = [CASES[k]() for k in sorted(CASES)]
+    specs = meshgen.batch_specs(nsys=3, lo=4, hi=8, seed=21)
+    systems += [meshgen.batch_system(sp) for sp in specs]
+    solvers = ["PCG" if s.symmetric else "PBiCGStab" for s in systems]
+    B = hip.Batch(systems, solvers)
+    b = np.concatenate([s.b for s in systems])
+    x, perfs = B.solve(b, rel_tol=1e-8, max_iter=1000)
+    off = B.cell_off
+    for i, (s, kind) in enumerate(zip(systems, solvers)):
+        xi = x[off[i]:off[i + 1]]
+        assert perfs[i]["converged"], (i, kind, perfs[i])
+        M = oracle.matrix(s)
+        assert np.linalg.norm(s.b - M.spmv(xi)) <= 2e-8 * np.linalg.norm(s.b), (i, kind)
+        pre = "DIC" if kind == "PCG" else "DILU"
+        # error bound as for the other batches: within 100x the error of the
+        # oracle's FP64 solve to the same tolerance, or 1e-6
+        xo, _ = M.solve(kind, pre, s.b, rel_tol=1e-8, max_iter=1000)
+        x_ref, _ = M.solve(kind, pre, s.b, rel_tol=1e-13, max_iter=5000)
+        e_batch = np.linalg.norm(xi - x_ref) / np.linalg.norm(x_ref)
+        e_fp64 = np.linalg.norm(xo - x_ref) / np.linalg.norm(x_ref)
+        assert e_batch <= max(100 * e_fp64, 1e-6), (i, kind, e_batch, e_fp64)
