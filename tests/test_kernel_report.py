@@ -54,3 +54,13 @@ def test_time_table_last_solve(tmp_path):
     assert len(row) == 1, out
     cells = [c.strip() for c in row[0].strip("|").split("|")]
     assert cells[1] == "2" and abs(float(cells[3]) - 3.0) < 1e-9  # two launches, mean 3 us
+
+
+def test_one_off_cast_counted_per_launch():
+    # the FP64 -> FP32 source cast runs once a solve when the Krylov update
+    # supplies the FP32 residual: its bytes follow its launches, not the
+    # iteration count
+    lv = [(1000, 3000), (250, 900), (60, 150)]
+    name = "void unap_hip::kRows<unap_hip::BCopy<double, float> >(unap_hip::BCopy<double, float>, int, unap_hip::Skip)"
+    assert kernel_report.alg_bytes(name, lv, 83, 1) == 12 * 1000
+    assert kernel_report.alg_bytes(name, lv, 83) == 83 * 12 * 1000

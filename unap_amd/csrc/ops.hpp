@@ -782,7 +782,11 @@ __global__ void __launch_bounds__(1024) kSweepTail(SweepArgs<T, TV> a, int l0, i
 
 // ================================================== dense coarsest (FP64)
 // y = Ainv b for the coarsest level of the mixed-precision hierarchy: one
-// wavefront per row, coalesced row reads, FP64 accumulation.
+// wavefront per row, coalesced row reads, FP64 accumulation.  A lane reads
+// kDenseBatch of its elements (64 apart) before it adds them, in element
+// order; positions past the row's end re-read its last element and are not
+// added.
+constexpr int kDenseBatch = 8;
 template <typename TB>
 __global__ void __launch_bounds__(256) kDenseSolve(const double* __restrict__ Ainv, const TB* b, TB* y,
                                                    int n, Skip sk) {
@@ -792,7 +796,18 @@ __global__ void __launch_bounds__(256) kDenseSolve(const double* __restrict__ Ai
     if (row >= n) return;
     const double* a = Ainv + (size_t)row * n;
     double acc = 0.0;
-    for (int j = lane; j < n; j += 64) acc += a[j] * (double)b[j];
+    for (int j0 = lane; j0 < n; j0 += 64 * kDenseBatch) {
+        double av[kDenseBatch], bv[kDenseBatch];
+#pragma unroll
+        for (int u = 0; u < kDenseBatch; ++u) {
+            const int j = min(j0 + 64 * u, n - 1);
+            av[u] = a[j];
+            bv[u] = (double)b[j];
+        }
+#pragma unroll
+        for (int u = 0; u < kDenseBatch; ++u)
+            if (j0 + 64 * u < n) acc += av[u] * bv[u];
+    }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
     if (lane == 0) y[row] = (TB)acc;

@@ -28,9 +28,10 @@ def _levels(h):
     return [(int(n), int(f)) for n, f in h]
 
 
-def alg_bytes(kernel, lv, iters):
+def alg_bytes(kernel, lv, iters, launches=None):
     """Algorithmic bytes over the whole solve for one kernel name, or None
-    when the kernel moves no per-cell data (finalizers, state updates)."""
+    when the kernel moves no per-cell data (finalizers, state updates).
+    Kernels that do not run every iteration are counted per launch."""
     n0, f0 = lv[0]
     L = len(lv) - 1              # coarse levels; the last is the coarsest
     smoothed = lv[:L]            # finest + coarse levels 0 .. L-2
@@ -48,8 +49,8 @@ def alg_bytes(kernel, lv, iters):
         return iters * sum((s + 4) * (lv[i][0] + lv[i + 1][0]) for i in range(L))
     if "BProlong<float>" in k:
         return iters * sum((s + 4) * lv[i][0] + s * lv[i + 1][0] for i in range(L))
-    if "BCopy<double, float>" in k:
-        return iters * 12 * n0
+    if "BCopy<double, float>" in k:  # the FP32 source cast, when the solver does not supply it
+        return (iters if launches is None else launches) * 12 * n0
     if "BFill<float>" in k:
         return iters * 4 * n0
     if "kDenseSolve" in k:
@@ -109,7 +110,7 @@ def report(trace_csv, bench_json):
            "|---|---|---|---|---|---|---|"]
     tot_b = 0.0
     for name, (c, d) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
-        b = alg_bytes(name, lv, iters)
+        b = alg_bytes(name, lv, iters, c)
         if b is None:
             out.append(f"| {short(name)} | {c} | {d / 1e6:.2f} | {d / c / 1e3:.1f} | - | - | - |")
             continue
