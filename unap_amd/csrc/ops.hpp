@@ -114,7 +114,7 @@ struct BAmulDot {
     double* w;
     __device__ void operator()(int i, double (&a)[1]) const {
         const double wi = rowAmul<double, double>(A, i, p);
-        w[i] = wi;
+        stS(&w[i], wi);
         a[0] += wi * p[i];
     }
     __device__ void terms(int i, double (&t)[1]) const { t[0] = w[i] * p[i]; }
@@ -129,9 +129,9 @@ struct BPcgXR {
     float* r32 = nullptr;  // also store r in FP32 (the input of an FP32 preconditioner)
     __device__ void operator()(int i, double (&a)[1]) const {
         const double al = st->alpha;
-        x[i] += al * p[i];
-        const double ri = r[i] - al * w[i];
-        r[i] = ri;
+        stS(&x[i], ldS(&x[i]) + al * ldS(&p[i]));
+        const double ri = ldS(&r[i]) - al * ldS(&w[i]);
+        stS(&r[i], ri);
         if (r32) r32[i] = (float)ri;
         a[0] += ri * ri;
     }
@@ -226,9 +226,9 @@ struct BFcgRz {
     const TZ* zOld;
     int first;
     __device__ void operator()(int i, double (&a)[2]) const {
-        const double ri = r[i];
-        a[0] += ri * (double)z[i];
-        if (!first) a[1] += ri * (double)zOld[i];
+        const double ri = ldS(&r[i]);
+        a[0] += ri * (double)ldS(&z[i]);
+        if (!first) a[1] += ri * (double)ldS(&zOld[i]);
     }
     __device__ void terms(int i, double (&t)[2]) const {
         t[0] = r[i] * (double)z[i];
@@ -242,9 +242,9 @@ struct BFcgP {
     const KState* st;
     __device__ void operator()(int i) const {
         if (st->nIter == 0)
-            p[i] = (double)z[i];
+            stS(&p[i], (double)ldS(&z[i]));
         else
-            p[i] = (double)z[i] + st->beta * p[i];
+            stS(&p[i], (double)ldS(&z[i]) + st->beta * ldS(&p[i]));
     }
 };
 
@@ -258,9 +258,9 @@ struct BSfAmul {
     T* Acf;
     __device__ void operator()(int i, double (&a)[2]) const {
         const T v = rowAmul<T, T, T, Shape>(A, i, f);
-        Acf[i] = v;
+        stL<Shape>(&Acf[i], v);
         const T fi = f[i];
-        a[0] += (double)(src[i] * fi);
+        a[0] += (double)(ldL<Shape>(&src[i]) * fi);
         a[1] += (double)(v * fi);
     }
     __device__ void terms(int i, double (&t)[2]) const {
@@ -398,15 +398,16 @@ struct BCheby {
     int sweep;
     __device__ void operator()(int i) const {
         const T ax = rowAmul<T, T, T, Shape>(A, i, xin);
-        const T rd = (T)1 / A.diag[i];
+        const T rd = (T)1 / ldL<Shape>(&A.diag[i]);
+        const T bi = ldL<Shape>(&b[i]);
         T wi;
         if (sweep == 0) {
-            wi = rd * (b[i] - ax) * (T)est->cheb[0];
+            wi = rd * (bi - ax) * (T)est->cheb[0];
         } else {
-            wi = (T)est->cheb[3 * sweep + 1] * w[i] + (T)est->cheb[3 * sweep + 2] * rd * (b[i] - ax);
+            wi = (T)est->cheb[3 * sweep + 1] * ldL<Shape>(&w[i]) + (T)est->cheb[3 * sweep + 2] * rd * (bi - ax);
         }
-        w[i] = wi;
-        xout[i] = xin[i] + wi;
+        stL<Shape>(&w[i], wi);
+        stL<Shape>(&xout[i], xin[i] + wi);
     }
 };
 
