@@ -650,8 +650,8 @@ __global__ void __launch_bounds__(256) kGsLevelLanes(SweepArgs<T, T> a, int p0, 
 // initial value (GS: b; forward: rD r; backward: w), scale (rD), diagonal
 // (GS), and the columns and coefficients of its first kUnroll entries.  The
 // tail loads a level's rows before the barrier that ends the level before it,
-// so what a level waits for after the barrier is only the gathers of values
-// that level wrote.  The arithmetic is sweepRow's, entry for entry.
+// so after that barrier the level waits only for its gathers of the values
+// the earlier levels wrote.  The arithmetic is sweepRow's, entry for entry.
 template <typename T, typename TV>
 struct TailRow {
     int r = -1;  // row written; -1: none
