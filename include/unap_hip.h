@@ -217,12 +217,15 @@ int unap_hip_solver_set_precond(unap_hip_solver S, unap_hip_precond P);
 int unap_hip_solver_set_limits(unap_hip_solver S, int64_t max_iter, int64_t min_iter,
                                double rel_tol, double tol);
 /* MGSolver(A, agglomeration, smoothers) with SET_nPreSweeps/nPostSweeps/
-   nFinestSweeps; weights NULL -> |upper| / ||upper||_2 (fortranInterface.cpp:383-395). */
+   nFinestSweeps; weights NULL -> |upper| / ||upper||_2 (fortranInterface.cpp:383-395).
+   The weights (one per face) are copied at the next solve: the array must
+   stay valid until then, and later solves need a matrix with that face count. */
 int unap_hip_solver_set_mg(unap_hip_solver S, const unap_hip_amg_params* amg, const double* weights);
 /* solve(x, A, b): x is the initial guess and the result. */
 int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const double* b, int where,
                           unap_hip_perf* perf);
-/* Residual history of the last solve: initial residual, then one per iteration. */
+/* Residual history of the last solve: initial residual, then one per iteration
+   (at most 2^20 + 2 entries are kept, whatever max_iter is). */
 int unap_hip_solver_history(unap_hip_solver S, double* out, int64_t cap, int64_t* len);
 /* Device time of the last solve in milliseconds (HIP events around the solve). */
 int unap_hip_solver_last_ms(unap_hip_solver S, double* ms);

@@ -17,7 +17,7 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 _f64p = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
 
-PRECOND = {"none": 0, "diag": 1, "DIC": 2, "DILU": 3, "AMG": 4}
+PRECOND = {"none": 0, "diag": 1, "DIC": 2, "DILU": 3, "AMG": 4, "GS1": 5}  # GS1: oracle only
 SOLVER = {"PCG": 1, "PBiCGStab": 2, "FCG": 4}
 SMOOTHER = {"GS": 1, "Cheby": 2}
 

@@ -37,6 +37,7 @@ Precond* makePrecond(const Ldu& A, int type) {
         case 1: return new DiagPrecond(A);
         case 2: return new DICPrecond(A);
         case 3: return new DILUPrecond(A);
+        case 5: return new GSPrecond(A);
         default: return new Precond;
     }
 }

@@ -32,7 +32,8 @@ void OFN(owner_start)(void* A, int64_t* out);   /* n+1 */
 void OFN(losort)(void* A, int64_t* out);        /* nf */
 void OFN(losort_start)(void* A, int64_t* out);  /* n+1 */
 
-/* type: 0 identity, 1 diagonal, 2 DIC, 3 DILU */
+/* type: 0 identity, 1 diagonal, 2 DIC, 3 DILU, 5 one forward Gauss-Seidel
+   sweep from zero (oracle only: a non-symmetric linear preconditioner) */
 void* OFN(precond_create)(void* A, int type);
 void OFN(precond_apply)(void* P, const double* r, double* w, int64_t n);
 void OFN(precond_rd)(void* P, double* out, int64_t n);
@@ -60,7 +61,8 @@ void OFN(aggl_coarse_matrix)(void* G, int64_t lev, int64_t* l, int64_t* u, doubl
 void OFN(aggl_destroy)(void* G);
 
 /* Krylov solve.  solver: 1 PCG, 2 PBiCGStab, 4 FCG (oracle only).
-   precond: 0 identity, 1 diag, 2 DIC, 3 DILU, 4 AMG (mg[] used).
+   precond: 0 identity, 1 diag, 2 DIC, 3 DILU, 4 AMG (mg[] used), 5 one
+   Gauss-Seidel sweep (oracle only).
    mg[] = {smootherType, nCoarsest, nVcycles, nPre, nPost, nFinest} (may be NULL
    unless precond == 4).  history (may be NULL) receives the initial residual
    then one residual per iteration, at most histCap values; the count is

@@ -185,6 +185,12 @@ void GSSmoother::smooth(double* x, const Ldu& A, const double* b, label nSweeps)
     }
 }
 
+// w = 0, then one GSSmoother sweep of A w = r.
+void GSPrecond::apply(double* w, const double* r, label n) const {
+    for (label i = 0; i < n; ++i) w[i] = 0.0;
+    GSSmoother().smooth(w, *A, r, 1);
+}
+
 // src/matrix/eigenvalue/eigenDiagPCG/eigenDiagPCG.cpp:48-212 (PCG loops) and
 // :213-308 (tridiagonal, Gershgorin range, Sturm bisection)
 double eigenDiagPCG(const Ldu& A, double* x, const double* b, const double* rD, label nPCGs,
@@ -598,31 +604,36 @@ Perf PBiCGStab::solve(double* x, const Ldu& A, const double* b) const {
 }
 
 // Flexible CG (no reference counterpart): PCG's loop shape and stopping test,
-// with beta = r_k.(z_k - z_{k-1}) / (r_{k-1}.z_{k-1}).
+// with the direction A-orthogonalised against the previous one (Notay 2000,
+// FCG(1); Golub and Ye 1999):
+//   beta_k = -(z_k . q_k-1) / (p_k-1 . q_k-1),   q_k-1 = A p_k-1,
+// which in exact arithmetic equals z_k . (r_k - r_k-1) / (r_k-1 . z_k-1) and
+// makes p_k . A p_k-1 = 0 for any preconditioner (non-symmetric or
+// iteration-dependent).  For a symmetric linear preconditioner it reduces to
+// PCG's beta = rho_k / rho_k-1 (PCG.cpp:120-134).
 Perf FCG::solve(double* x, const Ldu& A, const double* b) const {
     Perf perf;
     const label n = A.n;
-    std::vector<double> p(n, 0.0), q(n, 0.0), r(n, 0.0), z(n, 0.0), zOld(n, 0.0);
+    std::vector<double> p(n, 0.0), q(n, 0.0), r(n, 0.0), z(n, 0.0);
     A.spmv(q.data(), x);
     for (label i = 0; i < n; ++i) r[i] = b[i] - q[i];
     perf.initial = normFactor(r.data(), n);
     perf.final_ = perf.initial;
     if (history) history->assign(1, perf.initial);
-    double rz = GREAT_, rzOld;
+    double pq = 0.0;
     do {
-        rzOld = rz;
         M->apply(z.data(), r.data(), n);
-        rz = dot(r.data(), z.data(), n);
+        const double rz = dot(r.data(), z.data(), n);
         if (checkSingularity(perf, std::fabs(rz))) break;
         if (perf.nIter == 0) {
             for (label i = 0; i < n; ++i) p[i] = z[i];
         } else {
-            const double rzo = dot(r.data(), zOld.data(), n);
-            const double beta = (rz - rzo) / rzOld;
+            const double zq = dot(z.data(), q.data(), n);  // q = A p of the last iteration
+            const double beta = -zq / pq;
             for (label i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
         }
         A.spmv(q.data(), p.data());
-        const double pq = dot(p.data(), q.data(), n);
+        pq = dot(p.data(), q.data(), n);
         const double alpha = rz / pq;
         for (label i = 0; i < n; ++i) {
             x[i] += alpha * p[i];
@@ -630,7 +641,6 @@ Perf FCG::solve(double* x, const Ldu& A, const double* b) const {
         }
         perf.final_ = normFactor(r.data(), n);
         if (history) history->push_back(perf.final_);
-        z.swap(zOld);
     } while (++perf.nIter < maxIter && !checkConvergence(perf, tol, relTol, perf.nIter, minIter));
     return perf;
 }

@@ -84,6 +84,17 @@ struct DILUPrecond : Precond {
     const double* rD() const override { return rd.data(); }
 };
 
+// One forward Gauss-Seidel sweep from zero as a preconditioner,
+// w = (D + L)^-1 r: a linear but non-symmetric preconditioner built from the
+// reference's smoother (lduGaussSeidelSmoother.cpp:3-80).  Test-only: it
+// exercises the flexible-CG direction update, which a symmetric
+// preconditioner cannot tell apart from PCG's.
+struct GSPrecond : Precond {
+    const Ldu* A;
+    explicit GSPrecond(const Ldu& a) : A(&a) {}
+    void apply(double* w, const double* r, label n) const override;
+};
+
 // smoothers
 struct Smoother {
     virtual ~Smoother() {}
@@ -155,8 +166,9 @@ struct PBiCGStab : Solver {
     Perf solve(double* x, const Ldu& A, const double* b) const override;
 };
 
-// Flexible CG (Notay 2000, Polak-Ribiere beta): the outer loop of the
-// mixed-precision path; no counterpart in the reference.
+// Flexible CG (Notay 2000 FCG(1): each direction A-orthogonalised against the
+// previous one): the outer loop of the mixed-precision path; no counterpart in
+// the reference.
 struct FCG : Solver {
     const Precond* M;
     explicit FCG(const Precond& m) : M(&m) {}

@@ -253,15 +253,12 @@ def test_history_sensitivity(oracle):
 
 def test_fcg_equals_pcg_for_linear_preconditioner(oracle):
     """Flexible CG has no reference counterpart.  With a fixed SPD linear
-    preconditioner (DIC) its extra term r_k . z_k-1 vanishes in exact
-    arithmetic and it is PCG: same iteration count, residual histories
-    agreeing until rounding differences grow, same solution to tolerance;
-    and it matches a dense solve.  Taking z_k for z_k-1 (beta = 0, steepest
+    preconditioner (DIC) its beta = -(z_k . A p_k-1) / (p_k-1 . A p_k-1)
+    equals PCG's rho_k / rho_k-1 in exact arithmetic: same iteration count,
+    residual histories agreeing until rounding differences grow, same
+    solution to tolerance; and it matches a dense solve.  beta = 0 (steepest
     descent) or a wrong denominator breaks the iteration count and the
-    history.  The sign and presence of the r_k . z_k-1 term only matter under
-    a nonlinear preconditioner, which the oracle does not have: that part
-    stays unpinned here and is covered by the accuracy checks of the
-    mixed-precision solve against the FP64 solution."""
+    history.  The non-symmetric case is pinned by the next test."""
     s = meshgen.config_system(2, 10)
     M = oracle.matrix(s)
     xf, pf = M.solve("FCG", "DIC", s.b, rel_tol=1e-10, max_iter=500)
@@ -274,3 +271,84 @@ def test_fcg_equals_pcg_for_linear_preconditioner(oracle):
     s6 = meshgen.config_system(2, 6, seed=63)
     x6, p6 = oracle.matrix(s6).solve("FCG", "DIC", s6.b, rel_tol=1e-13, max_iter=1000)
     assert p6["converged"] and np.allclose(x6, np.linalg.solve(s6.dense(), s6.b), rtol=1e-9, atol=1e-12)
+
+
+def _serial_dot(a, b):
+    """The reference's left-to-right sum (unapVectorOperations.cpp:21-30)."""
+    return np.cumsum(a * b)[-1]
+
+
+def _fcg_emulation(M, s, beta_form, max_iter=60, rel_tol=1e-10):
+    """Flexible CG in numpy with z = one forward Gauss-Seidel sweep from zero
+    (oracle preconditioner "GS1", M = D + L: linear, not symmetric), the
+    oracle's operation order and serial dots.  Returns x, the residual
+    history and the search directions."""
+    x = np.zeros(s.n)
+    r = s.b - M.spmv(x)
+    init = np.sqrt(_serial_dot(r, r)) + 1e-37
+    hist, dirs = [init], []
+    p = q = None
+    pq = rz_old = z_old = r_old = None
+    it = 0
+    while True:
+        z, _ = M.precond("GS1", r)
+        rz = _serial_dot(r, z)
+        if it == 0:
+            p = z.copy()
+        else:
+            if beta_form == "notay":       # -(z_k . q_k-1) / (p_k-1 . q_k-1)
+                beta = -_serial_dot(z, q) / pq
+            elif beta_form == "notay_sign":  # planted mistake: sign flipped
+                beta = _serial_dot(z, q) / pq
+            else:                          # planted mistake: r_k . (z_k - z_k-1) / (r_k-1 . z_k-1)
+                beta = (rz - _serial_dot(r, z_old)) / rz_old
+            p = z + beta * p
+        q = M.spmv(p)
+        pq = _serial_dot(p, q)
+        alpha = rz / pq
+        dirs.append(p.copy())
+        x = x + alpha * p
+        r_old, r = r, r - alpha * q
+        hist.append(np.sqrt(_serial_dot(r, r)) + 1e-37)
+        z_old, rz_old = z, rz
+        it += 1
+        if not it < max_iter or hist[-1] <= rel_tol * init:
+            return x, np.array(hist), dirs
+
+
+def _max_a_cos(M, dirs):
+    """max_k |p_k . A p_k-1| / (||p_k||_A ||p_k-1||_A)."""
+    worst = 0.0
+    for a, b in zip(dirs[1:], dirs[:-1]):
+        Ab = M.spmv(b)
+        worst = max(worst, abs(a @ Ab) / np.sqrt((a @ M.spmv(a)) * (b @ Ab)))
+    return worst
+
+
+def test_fcg_a_orthogonal_under_nonsymmetric_preconditioner(oracle):
+    """Pins the flexible-CG direction update.  Under a non-symmetric
+    preconditioner (one forward Gauss-Seidel sweep, M = D + L) the
+    A-orthogonalised beta keeps every new direction A-orthogonal to the
+    previous one, a property of the mathematics (Notay 2000, Golub and Ye
+    1999) that the Polak-Ribiere form r_k . (z_k - z_k-1) / (r_k-1 . z_k-1)
+    and a sign error both violate.  The oracle's FCG with the same
+    preconditioner then reproduces the emulation bit for bit (history and
+    solution), and the solution matches a dense solve.  (C1 coefficients:
+    on the log-uniform ones this one-sweep preconditioner is too far from
+    symmetric for a one-direction flexible CG to converge.)"""
+    s = meshgen.config_system(1, 8, seed=71)
+    M = oracle.matrix(s)
+    xe, he, dirs = _fcg_emulation(M, s, "notay")
+    assert _max_a_cos(M, dirs) <= 1e-10
+    for bad in ("polak_ribiere", "notay_sign"):
+        _, _, dbad = _fcg_emulation(M, s, bad, max_iter=len(dirs))
+        assert _max_a_cos(M, dbad) > 1e-3, bad
+    xo, po = M.solve("FCG", "GS1", s.b, rel_tol=1e-10, max_iter=60)
+    assert po["nIterations"] == len(he) - 1
+    assert np.array_equal(po["history"], he)
+    assert np.array_equal(xo, xe)
+    assert po["converged"]
+    assert np.allclose(xo, np.linalg.solve(s.dense(), s.b), rtol=1e-8, atol=1e-10)
+    # the same preconditioner under PCG, which assumes symmetry, stalls
+    _, pp = M.solve("PCG", "GS1", s.b, rel_tol=1e-10, max_iter=400)
+    assert not pp["converged"]

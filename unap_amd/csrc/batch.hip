@@ -265,44 +265,40 @@ __device__ void batchPCG(const BatchArgs& a, const SysView& s, int off, unap_hip
     const double* b = a.b + off;
     double* r = a.v + off;
     double* p = a.v + a.total + off;
-    double* w = a.v + 2 * a.total + off;
-    double* zo = a.v + 3 * a.total + off;  // previous preconditioned residual
+    double* w = a.v + 2 * a.total + off;  // q = A p, kept into the next iteration's beta
     float* w32 = a.w32 + off;
     double acc = 0.0;
     for (int i = t; i < n; i += nt) {
         const double ri = b[i] - ellAmul(s.A, i, x);
         r[i] = ri;
         p[i] = 0.0;
-        zo[i] = 0.0;
+        w[i] = 0.0;
         acc += ri * ri;
     }
     const double init = sqrt(bsum(acc, red)) + kSmallD;
-    double fin = init, prev = init, rho = 1.0e37;
+    double fin = init, prev = init, rho = 1.0e37, pq = 1.0;
     long long it = 0;
     int conv = 0, sing = 0;
-    // Flexible CG: beta = r_k.(z_k - z_k-1) / (r_k-1.z_k-1), the form that
-    // tolerates a preconditioner evaluated in FP32 (not exactly linear).
+    // Flexible CG (Notay 2000, FCG(1)): beta = -(z_k . q_k-1) / (p_k-1 . q_k-1)
+    // makes each direction A-orthogonal to the previous one for a
+    // preconditioner evaluated in FP32 (not exactly linear or symmetric).
     for (;;) {
-        const double rhoOld = rho;
         batchPrecond(s, r, w32);
-        double rz = 0.0, rzo = 0.0;
-        vecRows<V3>(t, n, nt, [&](int i) { return V3{(double)w32[i], r[i], zo[i]}; },
+        double rz = 0.0, zq = 0.0;
+        vecRows<V3>(t, n, nt, [&](int i) { return V3{(double)w32[i], r[i], w[i]}; },
                     [&](int, const V3& v) {
                         rz += v.a * v.b;
-                        rzo += v.c * v.b;
+                        zq += v.a * v.c;
                     });
-        bsum2(rz, rzo, red);
+        bsum2(rz, zq, red);
         rho = rz;
         if (!(fabs(rho) > kSmallD)) {
             sing = 1;
             break;
         }
-        const double beta = (rz - rzo) / rhoOld;
+        const double beta = -zq / pq;
         vecRows<V2>(t, n, nt, [&](int i) { return V2{(double)w32[i], p[i]}; },
-                    [&](int i, const V2& v) {
-                        p[i] = (it == 0) ? v.a : v.a + beta * v.b;
-                        zo[i] = v.a;
-                    });
+                    [&](int i, const V2& v) { p[i] = (it == 0) ? v.a : v.a + beta * v.b; });
         __syncthreads();
         acc = 0.0;
         for (int i = t; i < n; i += nt) {
@@ -310,7 +306,8 @@ __device__ void batchPCG(const BatchArgs& a, const SysView& s, int off, unap_hip
             w[i] = v;
             acc += v * p[i];
         }
-        const double alpha = rho / bsum(acc, red);
+        pq = bsum(acc, red);
+        const double alpha = rho / pq;
         acc = 0.0;
         vecRows<V4>(t, n, nt, [&](int i) { return V4{x[i], p[i], r[i], w[i]}; },
                     [&](int i, const V4& v) {
@@ -582,8 +579,17 @@ int unap_hip_batch_create(int64_t nsys, const int64_t* cellOff, const int64_t* f
                           const int64_t* ua, const double* lower, const double* diag, const double* upper,
                           const int32_t* solver, unap_hip_batch* out) {
     return bguard([&] {
-        if (!(nsys > 0 && cellOff && faceOff && la && ua && diag && upper && solver && out))
+        if (!(nsys > 0 && nsys < (1LL << 31) && cellOff && faceOff && la && ua && diag && upper && solver && out))
             throw Error(UNAP_HIP_ERR_ARG, "null or empty batch argument");
+        // The device arrays hold cellOff[nsys] cells and faceOff[nsys] faces
+        // from offset 0, indexed with 32-bit ints.
+        if (cellOff[0] != 0 || faceOff[0] != 0)
+            throw Error(UNAP_HIP_ERR_ARG, "cell_off[0] and face_off[0] must be 0");
+        for (int64_t s = 0; s < nsys; ++s)
+            if (cellOff[s + 1] <= cellOff[s] || faceOff[s + 1] < faceOff[s])
+                throw Error(UNAP_HIP_ERR_ARG, "batch offsets must increase (cells) and not decrease (faces)");
+        if (cellOff[nsys] >= (1LL << 31) || faceOff[nsys] >= (1LL << 31))
+            throw Error(UNAP_HIP_ERR_ARG, "a batch holds fewer than 2^31 cells and 2^31 faces");
         Ctx& c = Ctx::get();
         auto* B = new unap_hip_batch_s;
         std::unique_ptr<unap_hip_batch_s> guardB(B);

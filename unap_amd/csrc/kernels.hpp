@@ -44,7 +44,7 @@ struct DMat {
 
 // Solver scalar state (one per running Krylov / eigen / MG context).
 struct KState {
-    double rho, rhoOld, alpha, beta, omega, rzo;
+    double rho, rhoOld, alpha, beta, omega, pq;
     double initRes, finalRes, prevRes;
     double tol, relTol;
     long long maxIter, minIter, nIter;
@@ -298,7 +298,7 @@ static __global__ void kStateInit(KState* st, double tol, double relTol, long lo
     if (threadIdx.x != 0) return;
     st->rho = 1.0e37;  // GREAT (PCG.cpp:57)
     st->rhoOld = 1.0e37;
-    st->alpha = st->beta = st->omega = st->rzo = 0.0;
+    st->alpha = st->beta = st->omega = st->pq = 0.0;
     st->initRes = st->finalRes = st->prevRes = 0.0;
     st->tol = tol;
     st->relTol = relTol;
@@ -389,18 +389,27 @@ struct FinBicgOmega {  // PBiCGStab.cpp:357-386: omega = (t.s) / (t.t)
 };
 
 // ---------------------------------------------------------------- FCG steps
-struct FinFcgRz {  // rz = r.z, rzo = r.z_old; beta = (rz - rzo) / rz_old
+// Flexible CG (Notay 2000, FCG(1)): rho = r.z and zq = z.q with q = A p of the
+// previous iteration; beta = -zq / (p.q), which A-orthogonalises the new
+// direction against the previous one for any preconditioner.
+struct FinFcgRz {
     KState* st;
     __device__ void operator()(double (&t)[2]) const {
         st->rhoOld = st->rho;
         st->rho = t[0];
-        st->rzo = t[1];
         st->singular = (fabs(st->rho) > kSmallD) ? 0 : 1;
         if (st->singular) {
             st->done = 1;
             return;
         }
-        if (st->nIter != 0) st->beta = (st->rho - st->rzo) / st->rhoOld;
+        if (st->nIter != 0) st->beta = -t[1] / st->pq;
+    }
+};
+struct FinFcgAlpha {  // pq = p.q kept for the next beta; alpha = rho / pq
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        st->pq = t[0];
+        st->alpha = st->rho / t[0];
     }
 };
 

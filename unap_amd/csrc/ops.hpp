@@ -218,21 +218,22 @@ static __global__ void kBicgEarlyX(double* x, const double* y, const KState* st,
 }
 
 // ================================================================ FCG bodies
-// rz = r.z and rzo = r.z_old with z in the preconditioner's precision
+// rho = r.z and zq = z.q (q = A p of the previous iteration), z in the
+// preconditioner's precision
 template <typename TZ>
 struct BFcgRz {
     const double* r;
     const TZ* z;
-    const TZ* zOld;
+    const double* q;
     int first;
     __device__ void operator()(int i, double (&a)[2]) const {
-        const double ri = ldS(&r[i]);
-        a[0] += ri * (double)ldS(&z[i]);
-        if (!first) a[1] += ri * (double)ldS(&zOld[i]);
+        const double zi = (double)ldS(&z[i]);
+        a[0] += ldS(&r[i]) * zi;
+        if (!first) a[1] += zi * ldS(&q[i]);
     }
     __device__ void terms(int i, double (&t)[2]) const {
         t[0] = r[i] * (double)z[i];
-        t[1] = first ? 0.0 : r[i] * (double)zOld[i];
+        t[1] = first ? 0.0 : (double)z[i] * q[i];
     }
 };
 template <typename TZ>

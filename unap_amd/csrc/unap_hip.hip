@@ -447,7 +447,7 @@ void applyPrecond64(PrecondObj& P, const double* r, double* w, Skip sk) {
 struct SolveBufs {
     int n = -1;
     DBuf<double> x, b, v[8];
-    DBuf<float> z32[2], r32;
+    DBuf<float> z32, r32;
     DBuf<double> hist;
     DBuf<KState> st;
     void ensure(int nn, long long histCap, bool f32) {
@@ -456,13 +456,11 @@ struct SolveBufs {
             b.alloc(nn);
             for (auto& q : v) q.alloc(nn);
             n = nn;
-            z32[0].release();
-            z32[1].release();
+            z32.release();
             r32.release();
         }
-        if (f32 && !z32[0].p) {
-            z32[0].alloc(nn);
-            z32[1].alloc(nn);
+        if (f32 && !z32.p) {
+            z32.alloc(nn);
             r32.alloc(nn);
         }
         if ((long long)hist.n < histCap) hist.alloc(histCap);
@@ -533,8 +531,8 @@ struct Driver {
         } else if (S.P->lu32) {
             // FP32 sweeps between an FP32 copy of the input and the output
             launchRows(BCopy<double, float>{in, B.r32.get()}, n, sk);
-            S.P->applyLu32(B.r32.get(), B.z32[0].get(), sk);
-            launchRows(BCopy<float, double>{B.z32[0].get(), out}, n, sk);
+            S.P->applyLu32(B.r32.get(), B.z32.get(), sk);
+            launchRows(BCopy<float, double>{B.z32.get(), out}, n, sk);
         } else {
             applyPrecond64(*S.P, in, out, sk);
         }
@@ -565,23 +563,21 @@ struct Driver {
         });
     }
 
-    // r32: an FP32 copy of r kept up to date by the residual update (the
-    // input of an FP32 preconditioner), or null.
+    // Flexible CG (Notay 2000, FCG(1)); q = A p survives into the next
+    // iteration's beta.  r32: an FP32 copy of r kept up to date by the
+    // residual update (the input of an FP32 preconditioner), or null.
     template <typename TZ>
-    void fcgWith(TZ* z0, TZ* z1, float* r32, std::function<void(const double*, TZ*, Skip)> prec) {
+    void fcgWith(TZ* z, float* r32, std::function<void(const double*, TZ*, Skip)> prec) {
         double *x = B.x.get(), *b = B.b.get(), *r = B.v[0].get(), *q = B.v[1].get(), *p = B.v[2].get();
         launchReduce<1>(BResidual<double>{M.view(), x, b, r, nullptr}, n, M.iperm.get(), FinInitRes{st, 0});
         if (r32) launchRows(BCopy<double, float>{r, r32}, n);
         const Skip sk{&st->done, nullptr};
-        TZ* z = z0;
-        TZ* zo = z1;
         iterate([&](long long it) {
             prec(r, z, sk);
-            launchReduce<2>(BFcgRz<TZ>{r, z, zo, it == 0 ? 1 : 0}, n, M.iperm.get(), FinFcgRz{st}, sk);
+            launchReduce<2>(BFcgRz<TZ>{r, z, q, it == 0 ? 1 : 0}, n, M.iperm.get(), FinFcgRz{st}, sk);
             launchRows(BFcgP<TZ>{z, p, st}, n, sk);
-            launchReduce<1>(BAmulDot{M.view(), p, q}, n, M.iperm.get(), FinPcgAlpha{st}, sk);
+            launchReduce<1>(BAmulDot{M.view(), p, q}, n, M.iperm.get(), FinFcgAlpha{st}, sk);
             launchReduce<1>(BPcgXR{x, r, p, q, st, r32}, n, M.iperm.get(), FinResNorm{st}, sk);
-            std::swap(z, zo);
         });
     }
 
@@ -589,15 +585,15 @@ struct Driver {
         if (S.P && S.P->lu32) {
             PrecondObj* P = S.P;
             float* r32 = B.r32.get();
-            fcgWith<float>(B.z32[0].get(), B.z32[1].get(), r32,
+            fcgWith<float>(B.z32.get(), r32,
                            [P, r32](const double*, float* z, Skip sk) { P->applyLu32(r32, z, sk); });
         } else if (S.P && S.P->amg32) {
             Hierarchy<float>* H = S.P->amg32.get();
             float* r32 = B.r32.get();
-            fcgWith<float>(B.z32[0].get(), B.z32[1].get(), r32,
+            fcgWith<float>(B.z32.get(), r32,
                            [H, r32](const double* r, float* z, Skip sk) { H->precondition(r, z, sk, r32); });
         } else {
-            fcgWith<double>(B.v[3].get(), B.v[4].get(), nullptr,
+            fcgWith<double>(B.v[3].get(), nullptr,
                             [this](const double* r, double* z, Skip sk) { precondInto(r, z, sk); });
         }
     }
@@ -1351,9 +1347,14 @@ int unap_hip_solver_set_mg(unap_hip_solver S, const unap_hip_amg_params* amg, co
         S->o.mgFor = nullptr;
         S->o.mgHasWeights = weights != nullptr;
         S->o.mgWeights.clear();
-        S->o.mgWeightsPtr = weights;
+        S->o.mgWeightsPtr = weights;  // copied at the next solve, when the face count is known
     });
 }
+
+// Residual history entries kept on the device per solve: the initial residual
+// and one per iteration, up to this many (a bound on the buffer whatever
+// max_iter is).
+static constexpr long long kMaxHistory = 1LL << 20;
 
 int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const double* b, int where,
                           unap_hip_perf* perf) {
@@ -1372,10 +1373,17 @@ int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const do
         if (o.P) order = o.P->order;
         else if (o.type == UNAP_HIP_SOLVER_MG && o.mg.smoother == UNAP_HIP_SMOOTHER_GS) order = ORDER_SWEEP;
         DevMatrix<double>& M = L.dev64(order);
-        const long long histCap = o.maxIter + 2;
+        const long long histCap = std::min(o.maxIter, kMaxHistory) + 2;
         S->bufs.ensure(M.n, histCap, o.P && (o.P->amg32 || o.P->lu32));
+        if (o.type == UNAP_HIP_SOLVER_MG && o.mgHasWeights && o.mgWeightsPtr) {
+            // the caller's weights are read once, here, and kept
+            o.mgWeights.assign(o.mgWeightsPtr, o.mgWeightsPtr + L.host.nFaces());
+            o.mgWeightsPtr = nullptr;
+        }
+        if (o.type == UNAP_HIP_SOLVER_MG && o.mgHasWeights)
+            need((int64_t)o.mgWeights.size() == L.host.nFaces(), "MG weights were given for a matrix with another face count");
         if (o.type == UNAP_HIP_SOLVER_MG && (!o.mgH || o.mgFor != &L)) {
-            auto ag = agglFor(L, o.mgWeightsPtr, true, o.mg.maxLevels, o.mg.nCoarsest);
+            auto ag = agglFor(L, o.mgHasWeights ? o.mgWeights.data() : nullptr, true, o.mg.maxLevels, o.mg.nCoarsest);
             o.mgH.reset(new Hierarchy<double>);
             o.mgH->build(L, ag, o.mg);
             o.mgFor = &L;
