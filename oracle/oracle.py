@@ -69,6 +69,12 @@ class CpuSolvePath:
              [_vp, _i64p, _f64p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
               _f64p, _f64p, _f64p, _f64p, ctypes.c_int64, _i64p])
 
+        _sig(L, P + "ldu_set_coeffs", None, [_vp, _f64p, _f64p, _f64p])
+        _sig(L, P + "solver_create", _vp, [_vp, ctypes.c_int, ctypes.c_int, _i64p])
+        _sig(L, P + "solver_run", ctypes.c_int,
+             [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _f64p, _f64p, _f64p,
+              _f64p, ctypes.c_int64, _i64p])
+        _sig(L, P + "solver_destroy", None, [_vp])
         _sig(L, P + "timed_solve", ctypes.c_int,
              [_vp, ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, _f64p, _f64p,
               _f64p, _f64p])
@@ -124,6 +130,10 @@ class CpuMatrix:
         self.path.fn("precond_destroy")(P)
         return w, rd
 
+    def precond_obj(self, kind):
+        """A preconditioner built now and kept (apply(r) -> w)."""
+        return CpuPrecond(self, kind)
+
     def smooth(self, kind, x, b, nsweeps, calls=1):
         S = self.path.fn("smoother_create")(SMOOTHER[kind])
         x = np.array(x, np.float64)
@@ -170,6 +180,18 @@ class CpuMatrix:
         self.path.fn("aggl_destroy")(G)
         return levels
 
+    def set_coeffs(self, diag, upper, lower=None):
+        """setMatrixCoeffients: new values, same addressing (lower None: symmetric)."""
+        d = np.ascontiguousarray(diag, np.float64)
+        up = np.ascontiguousarray(upper, np.float64)
+        lo = None if lower is None else np.ascontiguousarray(lower, np.float64)
+        self._vals = (d, up, lo)  # the oracle restatement copies; the reference shim copies too
+        self.path.fn("ldu_set_coeffs")(self.h, _p(d), _p(up), _p(lo))
+
+    def persistent(self, solver, precond=None, mg=None):
+        """A solver whose preconditioner / hierarchy is built now and kept."""
+        return CpuPersistentSolver(self, solver, precond, mg)
+
     def solve(self, solver, precond, b, x0=None, max_iter=1000, min_iter=0, rel_tol=1e-6, tol=0.0,
               mg=None, hist_cap=2048):
         x = np.zeros(self.n) if x0 is None else np.array(x0, np.float64)
@@ -187,6 +209,56 @@ class CpuMatrix:
                                               _p(hist), hist_cap, _p(hl, _i64p))
         if rc != 0:
             raise RuntimeError(f"{self.path.prefix}solve failed: {rc}")
+        h = hist[: min(int(hl[0]), hist_cap)].copy()
+        return x, {"initial": perf[0], "final": perf[1], "nIterations": int(perf[2]),
+                   "converged": bool(perf[3]), "singular": bool(perf[4]), "history": h}
+
+
+class CpuPrecond:
+    def __init__(self, M, kind):
+        self.M = M
+        self.h = M.path.fn("precond_create")(M.h, PRECOND[kind])
+
+    def __del__(self):
+        try:
+            self.M.path.fn("precond_destroy")(self.h)
+        except Exception:
+            pass
+
+    def apply(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        w = np.empty(self.M.n)
+        self.M.path.fn("precond_apply")(self.h, _p(r), _p(w), self.M.n)
+        return w
+
+
+class CpuPersistentSolver:
+    def __init__(self, M, solver, precond, mg):
+        self.M = M
+        self.mga = None if mg is None else np.asarray(mg, np.int64)
+        kind = 3 if solver == "MG" else SOLVER[solver]
+        pre = 0 if precond is None else PRECOND[precond]
+        self.h = M.path.fn("solver_create")(M.h, kind, pre, _p(self.mga, _i64p))
+        if not self.h:
+            raise RuntimeError("solver_create failed")
+
+    def __del__(self):
+        try:
+            self.M.path.fn("solver_destroy")(self.h)
+        except Exception:
+            pass
+
+    def solve(self, b, x0=None, max_iter=1000, min_iter=0, rel_tol=1e-6, tol=0.0, hist_cap=2048):
+        M = self.M
+        x = np.zeros(M.n) if x0 is None else np.array(x0, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        perf = np.zeros(6)
+        hist = np.zeros(hist_cap)
+        hl = np.zeros(1, np.int64)
+        rc = M.path.fn("solver_run")(self.h, M.h, max_iter, min_iter, rel_tol, tol, _p(x), _p(b), _p(perf),
+                                     _p(hist), hist_cap, _p(hl, _i64p))
+        if rc != 0:
+            raise RuntimeError(f"solver_run failed: {rc}")
         h = hist[: min(int(hl[0]), hist_cap)].copy()
         return x, {"initial": perf[0], "final": perf[1], "nIterations": int(perf[2]),
                    "converged": bool(perf[3]), "singular": bool(perf[4]), "history": h}

@@ -221,6 +221,89 @@ int uo_mg_solve(void* Av, const int64_t* mg, const double* weights, int64_t maxI
     return 0;
 }
 
+void uo_ldu_set_coeffs(void* Av, const double* diag, const double* upper, const double* lower) {
+    Ldu& A = *(Ldu*)Av;
+    const int64_t nf = A.nFaces();
+    A.d.assign(diag, diag + A.n);
+    A.up.assign(upper, upper + nf);
+    A.symm = (lower == nullptr || lower == upper);
+    A.lo.clear();
+    if (!A.symm) A.lo.assign(lower, lower + nf);
+}
+
+// A persistent solver: preconditioner or MG hierarchy built once.
+struct SolverBox {
+    int type = 1;
+    std::unique_ptr<Precond> P;
+    std::unique_ptr<Solver> S;
+    // standalone MG
+    std::unique_ptr<Agglomeration> G;
+    std::vector<std::unique_ptr<Smoother>> sm;
+};
+
+void* uo_solver_create(void* Av, int solver, int precond, const int64_t* mg) {
+    const Ldu& A = *(Ldu*)Av;
+    std::unique_ptr<SolverBox> b(new SolverBox);
+    b->type = solver;
+    if (solver == 3) {
+        if (!mg) return nullptr;
+        std::vector<double> w(A.nFaces());
+        const double avg = std::sqrt(sumSqr(A.upper(), A.nFaces()));
+        for (int64_t f = 0; f < A.nFaces(); ++f) w[f] = std::fabs(A.upper()[f]) / avg;
+        b->G.reset(new Agglomeration(A));
+        b->G->nCoarsest = mg[1];
+        b->G->agglomerate(w.data());
+        std::vector<const Smoother*> sp;
+        for (int64_t i = 0; i < b->G->size(); ++i) {
+            b->sm.emplace_back(mg[0] == 2 ? (Smoother*)new ChebySmoother : (Smoother*)new GSSmoother);
+            sp.push_back(b->sm.back().get());
+        }
+        MGSolver* M = new MGSolver(A, *b->G, sp);
+        M->nPre = mg[3];
+        M->nPost = mg[4];
+        M->nFinest = mg[5];
+        b->S.reset(M);
+        return b.release();
+    }
+    if (precond == 4) {
+        if (!mg) return nullptr;
+        b->P.reset(new MGPrecond(A, (int)mg[0], mg[1], mg[2], mg[3], mg[4], mg[5]));
+    } else {
+        b->P.reset(makePrecond(A, precond));
+    }
+    if (solver == 1)
+        b->S.reset(new PCG(*b->P));
+    else if (solver == 2)
+        b->S.reset(new PBiCGStab(*b->P));
+    else if (solver == 4)
+        b->S.reset(new FCG(*b->P));
+    else
+        return nullptr;
+    return b.release();
+}
+
+int uo_solver_run(void* Sv, void* Av, int64_t maxIter, int64_t minIter, double relTol, double tol,
+                  double* x, const double* b, double* perf, double* history, int64_t histCap,
+                  int64_t* histLen) {
+    SolverBox& B = *(SolverBox*)Sv;
+    const Ldu& A = *(Ldu*)Av;
+    for (auto& s : B.sm) s->init();
+    if (MGPrecond* mp = dynamic_cast<MGPrecond*>(B.P.get())) mp->init();
+    B.S->maxIter = maxIter;
+    B.S->minIter = minIter;
+    B.S->relTol = relTol;
+    B.S->tol = tol;
+    std::vector<double> h;
+    B.S->history = &h;
+    Perf p = B.S->solve(x, A, b);
+    B.S->history = nullptr;
+    fillPerf(p, perf);
+    copyHistory(h, history, histCap, histLen);
+    return 0;
+}
+
+void uo_solver_destroy(void* S) { delete (SolverBox*)S; }
+
 int uo_timed_solve(void* Av, int solver, int precond, const int64_t* mg, int64_t k1, int64_t k2,
                    double relTol, double* x, const double* b, double* perf, double* seconds) {
     const Ldu& A = *(Ldu*)Av;

@@ -77,6 +77,24 @@ int OFN(mg_solve)(void* A, const int64_t* mg, const double* weights, int64_t max
                   int64_t minIter, double relTol, double tol, double* x, const double* b,
                   double* perf, double* history, int64_t histCap, int64_t* histLen);
 
+/* setMatrixCoeffients(diag, upper, lower) (lduMatrix.cpp:508-529): new values,
+   same addressing; lower == NULL or == upper keeps one coefficient array. */
+void OFN(ldu_set_coeffs)(void* A, const double* diag, const double* upper, const double* lower);
+
+/* A solver whose preconditioner (or MG hierarchy) is built once, from A's
+   values at creation, and kept across solves, so that a coefficient update in
+   between shows what the reference's objects keep (DIC/DILU/diagonal rD,
+   agglomeration maps) and what they recompute (Galerkin matrices on every
+   MGSolver::solve).  solver 1, 2, 4 with precond as krylov_solve, or 3 (MG,
+   weights |upper| / ||upper||_2); mg[] as there.  Each run re-arms the
+   Chebyshev smoothers (initSmoothers, MG.cpp:54-59) and returns the history
+   as krylov_solve does. */
+void* OFN(solver_create)(void* A, int solver, int precond, const int64_t* mg);
+int OFN(solver_run)(void* S, void* A, int64_t maxIter, int64_t minIter, double relTol, double tol,
+                    double* x, const double* b, double* perf, double* history, int64_t histCap,
+                    int64_t* histLen);
+void OFN(solver_destroy)(void* S);
+
 /* Krylov solves timed with a steady clock, for the bench's CPU-baseline leg:
    seconds[0] = preconditioner setup (including AMG agglomeration),
    seconds[1] = a first solve capped at k1 iterations (cold: the Chebyshev

@@ -332,6 +332,110 @@ int ur_timed_solve(void* Av, int solver, int precond, const int64_t* mg, int64_t
     return 0;
 }
 
+void ur_ldu_set_coeffs(void* Av, const double* diag, const double* upper, const double* lower) {
+    lduMatrix& A = *(lduMatrix*)Av;
+    const label n = A.size(), nf = A.upperAddr().size();
+    scalarField dg = wrap(diag, n), up = wrap(upper, nf);
+    if (lower == nullptr || lower == upper) {
+        A.setMatrixCoeffients(dg, up, up);
+    } else {
+        scalarField lo = wrap(lower, nf);
+        A.setMatrixCoeffients(dg, up, lo);
+    }
+}
+
+// A persistent solver over reference objects (see oracle_capi.h).
+struct RefSolverBox {
+    int type = 1;
+    std::unique_ptr<matrix::preconditioner> P;
+    std::unique_ptr<matrix::solver> S;
+    // standalone MG
+    std::unique_ptr<lduAgglomeration> G;
+    PtrList<matrix::smoother>* sm = nullptr;
+    MGSolver* mg = nullptr;
+    ~RefSolverBox() {
+        S.reset();
+        delete mg;
+        delete sm;
+    }
+    void arm() const {
+        if (mg) mg->initSmoothers();
+        if (const RefMGPrecond* p = dynamic_cast<const RefMGPrecond*>(P.get())) p->mg->initSmoothers();
+    }
+    matrix::solver& solver() const { return mg ? (matrix::solver&)*mg : *S; }
+};
+
+void* ur_solver_create(void* Av, int solver, int precond, const int64_t* mg) {
+    ensureMPI();
+    const lduMatrix& A = *(lduMatrix*)Av;
+    RefSolverBox* b = new RefSolverBox;
+    b->type = solver;
+    if (solver == 3) {
+        scalarField w(A.upper().size());
+        const scalar avg = A.upper().SumSqrt();
+        for (label i = 0; i < w.size(); ++i) w[i] = std::fabs(A.upper()[i]) / avg;
+        b->G.reset(new lduAgglomeration(A));
+        b->G->SET_nCellsInCoarsestLevel(mg[1]);
+        b->G->agglomerate(w);
+        b->sm = new PtrList<matrix::smoother>(b->G->size());
+        for (label i = 0; i < b->G->size(); ++i) {
+            if (mg[0] == 2)
+                b->sm->setLevel(i, *new chebySmoother);
+            else
+                b->sm->setLevel(i, *new lduGaussSeidelSmoother);
+        }
+        b->mg = new MGSolver(A, *b->G, *b->sm);
+        b->mg->SET_nPreSweeps(mg[3]);
+        b->mg->SET_nPostSweeps(mg[4]);
+        b->mg->SET_nFinestSweeps(mg[5]);
+        return b;
+    }
+    if (precond == 4)
+        b->P.reset(new RefMGPrecond(A, (int)mg[0], mg[1], mg[2], mg[3], mg[4], mg[5]));
+    else
+        b->P.reset(makePrecond(A, precond));
+    if (solver == 1)
+        b->S.reset(new PCG(*b->P));
+    else
+        b->S.reset(new PBiCGStab(*b->P));
+    return b;
+}
+
+// The history comes from reruns capped at 1..K iterations with the same
+// (persistent) objects, each from the same start and with re-armed smoothers.
+int ur_solver_run(void* Sv, void* Av, int64_t maxIter, int64_t minIter, double relTol, double tol,
+                  double* x, const double* b, double* perf, double* history, int64_t histCap,
+                  int64_t* histLen) {
+    const RefSolverBox& B = *(RefSolverBox*)Sv;
+    const lduMatrix& A = *(lduMatrix*)Av;
+    const int64_t n = A.size();
+    scalarField bb = wrap(b, n);
+    std::vector<double> x0(x, x + n);
+    matrix::solver& S = B.solver();
+    auto run = [&](int64_t cap, double* out) {
+        B.arm();
+        S.SET_maxIter(cap);
+        S.SET_minIter(minIter);
+        S.SET_relTol(relTol);
+        S.SET_tolerance(tol);
+        scalarField xx = wrap(x0.data(), n);
+        matrix::solverPerformance p = S.solve(xx, A, bb);
+        if (out) std::memcpy(out, xx.values(), sizeof(double) * n);
+        return p;
+    };
+    matrix::solverPerformance fin = run(maxIter, x);
+    fillPerf(fin, perf);
+    int64_t len = 0;
+    if (history && histCap > 0) {
+        history[len++] = fin.initialResidual();
+        for (int64_t k = 1; k <= fin.nIterations() && len < histCap; ++k) history[len++] = run(k, nullptr).finalResidual();
+    }
+    if (histLen) *histLen = history ? len : fin.nIterations() + 1;
+    return 0;
+}
+
+void ur_solver_destroy(void* S) { delete (RefSolverBox*)S; }
+
 int ur_mg_solve(void* Av, const int64_t* mg, const double* weights, int64_t maxIter,
                 int64_t minIter, double relTol, double tol, double* x, const double* b,
                 double* perf, double* history, int64_t histCap, int64_t* histLen) {

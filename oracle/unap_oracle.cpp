@@ -729,9 +729,9 @@ void MGSolver::vcycle(double* psi, const double* source, double* Apsi, double* f
     sm[0]->smooth(psi, *finest, source, nFinest);
 }
 
-// src/solvers/MG/MGSolve.cpp:17-187 (the Galerkin coefficients of :108-111
-// are recomputed per call in the reference with identical values; here they
-// are computed once, when the hierarchy is built)
+// src/solvers/MG/MGSolve.cpp:17-187, including the Galerkin coefficients of
+// every level recomputed from the current fine coefficients on each call that
+// cycles (:108-111)
 Perf MGSolver::solve(double* x, const Ldu& A, const double* b) const {
     Perf perf;
     const label n = A.n;
@@ -744,6 +744,7 @@ Perf MGSolver::solve(double* x, const Ldu& A, const double* b) const {
     if (history) history->assign(1, perf.initial);
     if (!checkConvergence(perf, tol, relTol, perf.nIter, minIter)) {
         const label L = ag->size();
+        ag->agglomerateAll();
         std::vector<std::vector<double>> corr(L), src(L);
         for (label lv = 0; lv < L; ++lv) {
             corr[lv].assign(ag->coarse[lv].n, 0.0);

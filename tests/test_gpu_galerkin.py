@@ -1,0 +1,147 @@
+"""Galerkin coarse matrices on the GPU and coefficient updates.
+
+* The device Galerkin product (segmented sums over the agglomeration maps) is
+  bit-identical to the oracle's restatement of lduAgglomerateMatrix.cpp:3-104
+  on every level, symmetric and asymmetric.
+* After unap_hip_ldu_set_coeffs, solvers and preconditioners behave as the
+  reference's objects do after setMatrixCoeffients (lduMatrix.cpp:508-529):
+  DIC/DILU keep their rD, AMG hierarchies keep their agglomeration maps and
+  recompute the Galerkin levels (MGSolve.cpp:108-111).  Checked bitwise in
+  reference order against the oracle's persistent solvers, whose behaviour is
+  pinned to the compiled reference by
+  tests/test_oracle_pinning.py::test_set_coeffs_matches_reference.
+"""
+import numpy as np
+import pytest
+
+from unap_amd import hip, meshgen
+
+pytestmark = pytest.mark.gpu
+
+
+def _new_coeffs(s, seed):
+    rng = np.random.default_rng(seed)
+    up = s.upper * rng.uniform(0.5, 1.0, s.nfaces)
+    lo = None if s.symmetric else s.lower * rng.uniform(0.5, 1.0, s.nfaces)
+    return s.diag.copy(), up, lo
+
+
+@pytest.mark.parametrize("c,N,nc", [(2, 12, 20), (3, 12, 20), (4, 16, 30), (2, 48, 500), (3, 40, 200)])
+def test_galerkin_bitexact(oracle, c, N, nc):
+    s = meshgen.config_system(c, N)
+    A = hip.LduMatrix.from_system(s)
+    w = np.abs(s.upper)
+    got = hip.Agglomeration(A, w, n_coarsest=nc).levels()
+    ref = oracle.matrix(s).agglomerate(w, n_coarsest=nc)
+    assert len(got) == len(ref) > 0
+    for lev, (g, r) in enumerate(zip(got, ref)):
+        for k in ("restrict", "face_restrict", "l", "u", "diag", "upper", "lower"):
+            assert np.array_equal(g[k], r[k]), (lev, k)
+        assert g["symmetric"] == r["symmetric"]
+
+
+def test_galerkin_follows_set_coeffs(oracle):
+    # an agglomeration handle re-forms its coarse values from the new fine ones
+    s = meshgen.config_system(3, 12)
+    A = hip.LduMatrix.from_system(s)
+    w = np.abs(s.upper)
+    G = hip.Agglomeration(A, w, n_coarsest=20)
+    G.levels()
+    d, up, lo = _new_coeffs(s, 3)
+    A.set_coeffs(d, up, lo)
+    M = oracle.matrix(s)
+    M.set_coeffs(d, up, lo)
+    for lev, (g, r) in enumerate(zip(G.levels(), M.agglomerate(w, n_coarsest=20))):
+        for k in ("diag", "upper", "lower"):
+            assert np.array_equal(g[k], r[k]), (lev, k)
+
+
+@pytest.fixture
+def reference_order():
+    hip.set_reduction_order(hip.REDUCE_REFERENCE)
+    yield
+    hip.set_reduction_order(hip.REDUCE_TREE)
+
+
+CASES = [("PCG", "AMG", [1, 15, 1, 0, 2, 2], 2), ("PCG", "AMG", [2, 15, 1, 0, 2, 2], 2),
+         ("MG", None, [1, 15, 1, 0, 2, 2], 2), ("MG", None, [1, 15, 1, 0, 2, 2], 3),
+         ("PCG", "DIC", None, 2), ("PBiCGStab", "DILU", None, 3), ("FCG", "AMG", [1, 15, 1, 0, 2, 2], 2)]
+
+
+def _gpu_solver(A, solver, pre, mg):
+    if solver == "MG":
+        amg = hip.amg_params(smoother=mg[0], n_cells_coarsest=mg[1], n_pre_sweeps=mg[3], n_post_sweeps=mg[4],
+                             n_finest_sweeps=mg[5])
+        return None, hip.Solver("MG", mg=amg, rel_tol=1e-8, max_iter=150)
+    amg = None
+    if pre == "AMG":
+        amg = hip.amg_params(smoother=mg[0], n_cells_coarsest=mg[1], n_vcycles=mg[2], n_pre_sweeps=mg[3],
+                             n_post_sweeps=mg[4], n_finest_sweeps=mg[5])
+    P = hip.Precond(A, pre, amg=amg)
+    return P, hip.Solver(solver, P, rel_tol=1e-8, max_iter=150)
+
+
+@pytest.mark.parametrize("solver,pre,mg,c", CASES)
+def test_set_coeffs_parity(oracle, reference_order, solver, pre, mg, c):
+    """create -> solve -> set_coeffs -> solve, bitwise against the oracle."""
+    s = meshgen.config_system(c, 10, seed=81)
+    A = hip.LduMatrix.from_system(s)
+    P, S = _gpu_solver(A, solver, pre, mg)
+    M = oracle.matrix(s)
+    So = M.persistent(solver, pre, mg)
+    new = _new_coeffs(s, 5)
+    for step in range(2):
+        x, p = S.solve(A, s.b)
+        xo, po = So.solve(s.b, rel_tol=1e-8, max_iter=150)
+        assert p["nIterations"] == po["nIterations"], (step, p["nIterations"], po["nIterations"])
+        assert np.array_equal(p["history"], po["history"]), step
+        assert np.array_equal(x, xo), step
+        if step == 0:
+            A.set_coeffs(*new)
+            M.set_coeffs(*new)
+
+
+def test_set_coeffs_mixed_fcg(oracle):
+    """The FP32 fixed-cycle hierarchy after a coefficient update: its FCG
+    solve meets the tolerance on the new system (Galerkin levels, dense
+    coarsest inverse and Chebyshev estimates all redone)."""
+    s = meshgen.config_system(4, 20)
+    A = hip.LduMatrix.from_system(s)
+    P = hip.Precond(A, "AMG", amg=hip.amg_params(smoother=2, n_cells_coarsest=50, n_vcycles=1, n_post_sweeps=4,
+                                                   n_finest_sweeps=4, precision=32, pair_steps=2,
+                                                   cycle=hip.CYCLE_FIXED))
+    S = hip.Solver("FCG", P, rel_tol=1e-8, max_iter=300)
+    x1, p1 = S.solve(A, s.b)
+    d, up, lo = _new_coeffs(s, 9)
+    up = up * 1e-2  # a strong change: the old hierarchy would be a poor preconditioner
+    A.set_coeffs(d, up, lo)
+    x2, p2 = S.solve(A, s.b)
+    M = oracle.matrix(s)
+    M.set_coeffs(d, up, lo)
+    r = s.b - M.spmv(x2)
+    assert p2["converged"] and np.linalg.norm(r) <= 2e-8 * np.linalg.norm(s.b)
+    # a hierarchy built fresh on the new values converges the same way
+    P2 = hip.Precond(A, "AMG", amg=hip.amg_params(smoother=2, n_cells_coarsest=50, n_vcycles=1, n_post_sweeps=4,
+                                                    n_finest_sweeps=4, precision=32, pair_steps=2,
+                                                    cycle=hip.CYCLE_FIXED))
+    x3, p3 = hip.Solver("FCG", P2, rel_tol=1e-8, max_iter=300).solve(A, s.b)
+    assert p3["converged"]
+
+
+def test_set_coeffs_symmetric_to_asymmetric_dic(oracle):
+    """DIC built on symmetric storage, then asymmetric coefficients: the
+    sweeps read upper on both sides (lduDICPrecond.cpp:68-76) with the rD of
+    creation, bit for bit with the oracle."""
+    s = meshgen.config_system(2, 12)
+    A = hip.LduMatrix.from_system(s)
+    P = hip.Precond(A, "DIC")
+    M = oracle.matrix(s)
+    Po = M.precond_obj("DIC")
+    r = np.random.default_rng(4).uniform(-1, 1, s.n)
+    assert np.array_equal(P.apply(r), Po.apply(r))
+    d, up, _ = _new_coeffs(s, 6)
+    lo = up * np.random.default_rng(7).uniform(0.5, 1.0, s.nfaces)
+    A.set_coeffs(d, up, lo)
+    M.set_coeffs(d, up, lo)
+    assert np.array_equal(P.apply(r), Po.apply(r))
+    assert np.array_equal(A.spmv(r), M.spmv(r))

@@ -2,7 +2,8 @@
 declare, refuses bad input with a status code, and its host-side setup (cell
 ordering, level schedules, pairwise agglomeration, Galerkin coarse matrices)
 matches the oracle and the compiled reference bit for bit.  No GPU compute
-call is made here: matrix and agglomeration handles build on the host."""
+call is made here: matrix and agglomeration handles, maps and coarse
+addressing build on the host (the Galerkin values need the GPU)."""
 import os
 import re
 
@@ -68,16 +69,21 @@ def test_level_schedule_and_order(c, N):
 @pytest.mark.parametrize("c,N,nc", [(2, 10, 20), (3, 10, 20), (2, 13, 50), (4, 12, 30), (1, 9, 10), (2, 48, 500),
                                     (3, 40, 200)])
 def test_agglomeration_matches_oracle(oracle, c, N, nc):
+    # maps and coarse addressing (host); the Galerkin values are formed on
+    # the GPU and compared in tests/test_gpu_galerkin.py
     s = meshgen.config_system(c, N)
     A = hip.LduMatrix.from_system(s)
     w = np.abs(s.upper)
-    got = hip.Agglomeration(A, w, n_coarsest=nc).levels()
-    ref = oracle.matrix(s).agglomerate(w, n_coarsest=nc)
+    got = hip.Agglomeration(A, w, n_coarsest=nc).levels(values=False)
+    ref = oracle.matrix(s).agglomerate(w, n_coarsest=nc, matrices=False)
     assert len(got) == len(ref) > 0
-    for g, r in zip(got, ref):
-        for k in ("restrict", "face_restrict", "l", "u", "diag", "upper", "lower"):
+    lref = oracle.matrix(s).agglomerate(w, n_coarsest=nc)
+    for g, r, rl in zip(got, ref, lref):
+        for k in ("restrict", "face_restrict"):
             assert np.array_equal(g[k], r[k]), k
-        assert g["symmetric"] == r["symmetric"]
+        for k in ("l", "u"):
+            assert np.array_equal(g[k], rl[k]), k
+        assert g["symmetric"] == rl["symmetric"]
 
 
 @pytest.mark.parametrize("c,N,seed", [(2, 11, 5), (3, 11, 6)])
@@ -85,9 +91,9 @@ def test_agglomeration_matches_reference(reference, c, N, seed):
     s = meshgen.config_system(c, N, seed=seed)
     A = hip.LduMatrix.from_system(s)
     w = np.abs(s.upper) / np.linalg.norm(s.upper)
-    got = hip.Agglomeration(A, w, n_coarsest=25).levels()
+    got = hip.Agglomeration(A, w, n_coarsest=25).levels(values=False)
     ref = reference.matrix(s).agglomerate(w, n_coarsest=25)
     assert len(got) == len(ref)
     for g, r in zip(got, ref):
-        for k in ("restrict", "face_restrict", "l", "u", "diag", "upper", "lower"):
+        for k in ("restrict", "face_restrict", "l", "u"):
             assert np.array_equal(g[k], r[k]), k

@@ -352,3 +352,46 @@ def test_fcg_a_orthogonal_under_nonsymmetric_preconditioner(oracle):
     # the same preconditioner under PCG, which assumes symmetry, stalls
     _, pp = M.solve("PCG", "GS1", s.b, rel_tol=1e-10, max_iter=400)
     assert not pp["converged"]
+
+
+def _new_coeffs(s, seed):
+    """Same addressing, new values: off-diagonals scaled by seeded factors in
+    [0.5, 1] (diagonal kept, so the system stays diagonally dominant)."""
+    rng = np.random.default_rng(seed)
+    up = s.upper * rng.uniform(0.5, 1.0, s.nfaces)
+    lo = None if s.symmetric else s.lower * rng.uniform(0.5, 1.0, s.nfaces)
+    return s.diag.copy(), up, lo
+
+
+SET_COEFF_CASES = [("PCG", "AMG", [1, 15, 1, 0, 2, 2], 2), ("PCG", "AMG", [2, 15, 1, 0, 2, 2], 2),
+                   ("MG", None, [1, 15, 1, 0, 2, 2], 2), ("MG", None, [1, 15, 1, 0, 2, 2], 3),
+                   ("PCG", "DIC", None, 2), ("PBiCGStab", "DILU", None, 3), ("FCG", "AMG", [1, 15, 1, 0, 2, 2], 2)]
+
+
+@pytest.mark.parametrize("solver,pre,mg,c", SET_COEFF_CASES)
+def test_set_coeffs_matches_reference(oracle, reference, solver, pre, mg, c):
+    """create -> solve -> setMatrixCoeffients -> solve with the solver and its
+    preconditioner kept: the reference keeps DIC/DILU rD and the
+    agglomeration maps from construction and recomputes the Galerkin
+    matrices in every MGSolver::solve (MGSolve.cpp:108-111).  The oracle does
+    the same, bit for bit.  (FCG has no reference counterpart: oracle only.)"""
+    s = meshgen.config_system(c, 10, seed=81)
+    paths = [oracle] if solver == "FCG" else [oracle, reference]
+    out = []
+    for path in paths:
+        M = path.matrix(s)
+        S = M.persistent(solver, pre, mg)
+        r1 = S.solve(s.b, rel_tol=1e-8, max_iter=150)
+        M.set_coeffs(*_new_coeffs(s, 5))
+        r2 = S.solve(s.b, rel_tol=1e-8, max_iter=150)
+        out.append((r1, r2))
+    for (xa, pa), (xb, pb) in zip(out[0], out[-1]):
+        assert pa["nIterations"] == pb["nIterations"]
+        assert np.array_equal(pa["history"], pb["history"])
+        assert np.array_equal(xa, xb)
+    # the second solve solves the new system
+    d, up, lo = _new_coeffs(s, 5)
+    s2 = meshgen.LduSystem(s.n, s.l, s.u, lo, d, up, s.b, s.symmetric)
+    x2, p2 = out[0][1]
+    assert p2["converged"]
+    assert np.linalg.norm(s2.dense() @ x2 - s.b) <= 2e-8 * np.linalg.norm(s.b) * 10

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "galerkin.hpp"
 #include "host_setup.hpp"
 #include "kernels.hpp"
 
@@ -162,6 +163,8 @@ struct DevMatrix {
     DBuf<T> val, diag;
     DBuf<T> valU;  // upper on both sides (DIC on asymmetric storage), lazily
     DBuf<int> perm, iperm;
+    DBuf<int> src;  // slot -> 2 face + side, kept for matrices filled on the device
+    long long nnzPadded = 0;
     DBuf<int> fwdRows, fwdStart, bwdRows, bwdStart;  // fwdRows empty: level-major order
     std::vector<int> fwdStartH, bwdStartH;
     bool sweepable = false;
@@ -202,7 +205,45 @@ struct DevMatrix {
         return {n, bwdSlice.get(), bwdInfo.get(), bwdCol.get(), bwdVal.get(), nullptr};
     }
 
+    // Values from host coefficients (external order).
     void build(const HostLdu& A, const Layout& L, hipStream_t s) {
+        buildStructure(L, s);
+        std::vector<T> v;
+        fillSellValues<T>(L, A, false, v);
+        val.upload(v, s);
+        std::vector<T> d;
+        fillDiag<T>(L, A, d);
+        diag.upload(d, s);
+        UH_CHECK(hipStreamSynchronize(s));  // host staging vectors die here
+    }
+    // Values from device coefficients in external order (a coarse level's
+    // Galerkin product); the slot map stays on the device for later refills.
+    void buildOnDevice(const Layout& L, const double* d, const double* up, const double* lo, hipStream_t s) {
+        buildStructure(L, s);
+        src.upload(L.src, s);
+        nnzPadded = L.nnzPadded;
+        fillOnDevice(d, up, lo, s);
+        UH_CHECK(hipStreamSynchronize(s));
+    }
+    // New values (same structure) from device coefficients in external order.
+    void fillOnDevice(const double* d, const double* up, const double* lo, hipStream_t s) {
+        if (!src.p && nnzPadded > 0) throw Error(-5, "device refill of a matrix built from host values");
+        if (val.n != (size_t)std::max<long long>(nnzPadded, 1)) val.alloc(std::max<long long>(nnzPadded, 1));
+        if (diag.n != (size_t)n) diag.alloc(n);
+        if (nnzPadded > 0) {
+            const long long g = std::min<long long>((nnzPadded + 255) / 256, 8192);
+            kFillSell<T><<<(unsigned)g, 256, 0, s>>>(nnzPadded, src.get(), up, lo, 0, val.get());
+            UH_LAUNCHED();
+        }
+        if (n > 0) {
+            kFillDiag<T><<<(n + 255) / 256, 256, 0, s>>>(n, perm.get(), d, diag.get());
+            UH_LAUNCHED();
+        }
+        valU.release();
+        bwdVal.release();  // rebuilt from the new values on the next backward sweep
+    }
+    // Addressing, schedules and the backward-copy structure; no values.
+    void buildStructure(const Layout& L, hipStream_t s) {
         n = L.n;
         sweepable = L.sweepable;
         shape = (2.0 * L.nf > 14.0 * L.n) ? kShapeBatch4 : kShapeWide8;
@@ -211,12 +252,6 @@ struct DevMatrix {
         sliceStart.upload(L.sliceStart, s);
         rowInfo.upload(L.rowInfo, s);
         col.upload(L.col, s);
-        std::vector<T> v;
-        fillSellValues<T>(L, A, false, v);
-        val.upload(v, s);
-        std::vector<T> d;
-        fillDiag<T>(L, A, d);
-        diag.upload(d, s);
         perm.upload(L.perm, s);
         iperm.upload(L.iperm, s);
         fwdStartH = L.fwdStart;
@@ -239,7 +274,6 @@ struct DevMatrix {
                 bwdSliceH[sl + 1] = bwdSliceH[sl] + (long long)w * kSliceRows;
             }
         }
-        UH_CHECK(hipStreamSynchronize(s));  // host staging vectors die here
     }
     void buildValU(const HostLdu& A, const Layout& L, hipStream_t s) {
         if (A.symm || valU.p) return;

@@ -546,49 +546,6 @@ void coarseAddressing(const HostLdu& F, CoarseLevel& C) {
         }
 }
 
-// Galerkin coarse coefficients (lduAgglomerateMatrix.cpp:3-104).
-void galerkin(const HostLdu& F, CoarseLevel& C) {
-    HostLdu& A = C.A;
-    const int ncf = A.nFaces();
-    A.d.assign(A.n, 0.0);
-    for (int i = 0; i < F.n; ++i) A.d[C.restrictMap[i]] += F.d[i];
-    const std::vector<int>& fr = C.faceRestrictMap;
-    if (F.symm) {
-        A.symm = true;
-        A.lo.clear();
-        A.up.assign(ncf, 0.0);
-        for (int f = 0; f < F.nFaces(); ++f) {
-            const int cf = fr[f];
-            if (cf >= 0)
-                A.up[cf] += F.up[f];
-            else
-                A.d[-1 - cf] += 2 * F.up[f];
-        }
-        return;
-    }
-    A.symm = false;
-    A.up.assign(ncf, 0.0);
-    A.lo.assign(ncf, 0.0);
-    const double* fl = F.lower();
-    for (int f = 0; f < F.nFaces(); ++f) {
-        const int cf = fr[f];
-        if (cf >= 0) {
-            const int cOwnOfFineOwner = C.restrictMap[F.l[f]];
-            if (A.l[cf] == cOwnOfFineOwner) {
-                A.up[cf] += F.up[f];
-                A.lo[cf] += fl[f];
-            } else if (A.u[cf] == cOwnOfFineOwner) {
-                A.up[cf] += fl[f];
-                A.lo[cf] += F.up[f];
-            } else {
-                throw std::runtime_error("agglomeration: inconsistent coarse addressing");
-            }
-        } else {
-            A.d[-1 - cf] += F.up[f] + fl[f];
-        }
-    }
-}
-
 }  // namespace
 
 std::vector<CoarseLevel> agglomerate(const HostLdu& fine, const std::vector<double>& weights,
@@ -608,7 +565,8 @@ std::vector<CoarseLevel> agglomerate(const HostLdu& fine, const std::vector<doub
         w.swap(cw);
         levels.push_back(std::move(C));
     }
-    for (size_t i = 0; i < levels.size(); ++i) galerkin(i == 0 ? fine : levels[i - 1].A, levels[i]);
+    // coefficients: on the device (galerkin.hpp), from the fine values of the time
+    for (auto& C : levels) C.A.symm = fine.symm;
     return levels;
 }
 

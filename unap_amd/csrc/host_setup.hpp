@@ -94,12 +94,15 @@ struct CoarseLevel {
     int nCoarse = 0;
     std::vector<int> restrictMap;      // fine external -> coarse external
     std::vector<int> faceRestrictMap;  // fine face -> coarse face, or -(coarse cell + 1)
-    HostLdu A;                         // coarse matrix (addressing + Galerkin values)
+    HostLdu A;                         // coarse matrix: addressing here; the Galerkin
+                                       // values are formed on the device and copied
+                                       // back when host code needs them
 };
 
 // lduAgglomeration::agglomerate (lduAgglomerate.cpp:4-91) with the pairwise
-// step (:94-271), coarse addressing (lduAgglomerationAddressing.cpp:8-178)
-// and Galerkin coefficients (lduAgglomerateMatrix.cpp:3-104).
+// step (:94-271) and coarse addressing (lduAgglomerationAddressing.cpp:8-178).
+// The Galerkin coefficients (lduAgglomerateMatrix.cpp:3-104) are not formed
+// here (galerkin.hpp).
 std::vector<CoarseLevel> agglomerate(const HostLdu& fine, const std::vector<double>& weights,
                                      int maxLevels, int nCellsInCoarsestLevel);
 

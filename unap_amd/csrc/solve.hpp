@@ -38,6 +38,16 @@ struct LduView {
 struct LduObj {
     HostLdu host;
     int nFwdLevels = 0, nBwdLevels = 0;
+    // bumped by every coefficient update: hierarchies built on this matrix
+    // recompute their Galerkin levels when it moves
+    long long coeffGen = 0;
+    // process-unique id (caches keyed on a matrix never confuse two matrices
+    // that lived at one address)
+    const unsigned long long id = nextId();
+    static unsigned long long nextId() {
+        static unsigned long long k = 0;
+        return ++k;
+    }
     // agglomeration built from the default weights, shared by every
     // preconditioner / solver of this matrix with the same level limits
     std::shared_ptr<AgglHost> agglCache;
@@ -227,8 +237,32 @@ struct SmootherObj {
 };
 
 // ----------------------------------------------------- agglomeration levels
+// Device side of one agglomeration level: the segmented-sum lists of the
+// Galerkin product (galerkin.hpp) and the coarse coefficients in the coarse
+// level's external order.
+struct AgglDevLevel {
+    DBuf<int> childStart, child;   // per coarse cell: fine cells, ascending
+    DBuf<int> innerStart, inner;   // per coarse cell: fine faces inside it, ascending
+    DBuf<int> cfStart, cfFine;     // per coarse face: fine faces, ascending (~f: swapped sides)
+    DBuf<double> d, up, lo;        // coarse diag, upper, lower (lo empty when symmetric)
+};
+
 struct AgglHost {
     std::vector<CoarseLevel> levels;
+    // Galerkin coefficients on the device for the fine values of generation
+    // valuesGen of the fine matrix (-1: not formed); hostValid[k]: level k's
+    // values are also in levels[k].A
+    std::vector<AgglDevLevel> dev;
+    DBuf<double> fineD, fineUp, fineLo;
+    long long valuesGen = -1;
+    std::vector<char> hostValid;
+    // Galerkin products of every level from the fine coefficients (fine.symm
+    // decides the symmetric or asymmetric form, as lduAgglomerateMatrix.cpp:29
+    // does per level).
+    void galerkin(const HostLdu& fine, long long gen);
+    // levels[k].A with its Galerkin values (copied from the device once per
+    // generation)
+    const HostLdu& hostLevel(int k);
     // device layouts of the coarse levels per layout mode, each built when a
     // consumer first asks for it (a hierarchy that keeps every second
     // agglomeration level never builds the others)
@@ -252,6 +286,9 @@ inline void buildAggl(const HostLdu& fine, const std::vector<double>& w, int max
     out.levels = agglomerate(fine, w, maxLevels, nCoarsest);
     for (auto& v : out.layouts) v.clear();
     for (auto& v : out.built) v.clear();
+    out.dev.clear();
+    out.valuesGen = -1;
+    out.hostValid.assign(out.levels.size(), 0);
 }
 
 // -------------------------------------------------------------- hierarchy
@@ -303,7 +340,16 @@ struct Hierarchy {
     int nl(int l) const { return l < 0 ? fine->n : lev(l).nCoarse; }
     DevMatrix<T>& mat(int l) { return l < 0 ? *fine : *mats[l]; }
 
+    // fine coefficient generation the coarse values and coarsest data follow
+    long long valuesGen = -1;
+
     void build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgParams& p);
+    // After a coefficient update of the fine matrix: the Galerkin levels again
+    // from the new fine values (MGSolve.cpp:108-111 does it on every solve),
+    // with the agglomeration maps kept, and the coarsest-level data and
+    // estimates derived from them.  No-op while the values are current.
+    void refresh();
+    void setupNumeric();
     // The reference re-arms the Chebyshev estimate on every solve; a fixed
     // cycle keeps the estimates taken at setup.
     void initSmoothers() {
@@ -333,6 +379,8 @@ struct PrecondObj {
     int precision() const { return (amg32 || lu32) ? 32 : 64; }
     // w = M^-1 r with FP32 sweeps (lu32), on FP32 vectors
     void applyLu32(const float* r, float* w, Skip sk) {
+        if (type == UNAP_HIP_PRECOND_DIC)  // upper on both sides once the storage is asymmetric
+            A->dev32(order).buildValU(A->host, A->layout(order), Ctx::get().stream);
         luSweeps<float, float>(A->dev32(order), type == UNAP_HIP_PRECOND_DIC, w, r, rD32.get(), sk, rD32pos.get());
     }
 };
@@ -353,7 +401,7 @@ struct SolverObj {
     bool mgHasWeights = false;
     const double* mgWeightsPtr = nullptr;  // caller-owned, read at the next solve
     std::unique_ptr<Hierarchy<double>> mgH;
-    LduObj* mgFor = nullptr;
+    unsigned long long mgFor = 0;  // id of the matrix mgH was built for
     // results
     std::vector<double> history;
     double lastMs = 0;

@@ -96,6 +96,134 @@ static std::vector<double> denseInverse(std::vector<double> a, int n) {
     return inv;
 }
 
+// ------------------------------------------------------ Galerkin on the GPU
+// The segmented-sum lists of every level are built from the host maps once;
+// each call then runs two kernels a level, level k's coarse values being the
+// fine values of level k + 1.
+void AgglHost::galerkin(const HostLdu& fine, long long gen) {
+    Ctx& c = Ctx::get();
+    const int K = (int)levels.size();
+    if ((int)dev.size() != K) {
+        dev.clear();
+        dev.resize(K);
+        for (int k = 0; k < K; ++k) {
+            const CoarseLevel& C = levels[k];
+            const std::vector<int>& fl = (k == 0) ? fine.l : levels[k - 1].A.l;
+            const int nfc = (int)C.restrictMap.size(), nff = (int)C.faceRestrictMap.size();
+            const int nc = C.nCoarse, ncf = C.A.nFaces();
+            std::vector<int> cst(nc + 1, 0), cl(nfc), ist(nc + 1, 0), il, fst(ncf + 1, 0), ff;
+            for (int i = 0; i < nfc; ++i) cst[C.restrictMap[i] + 1]++;
+            for (int f = 0; f < nff; ++f) {
+                const int cf = C.faceRestrictMap[f];
+                if (cf >= 0)
+                    fst[cf + 1]++;
+                else
+                    ist[-cf]++;
+            }
+            for (int q = 0; q < nc; ++q) {
+                cst[q + 1] += cst[q];
+                ist[q + 1] += ist[q];
+            }
+            for (int q = 0; q < ncf; ++q) fst[q + 1] += fst[q];
+            il.resize(ist[nc]);
+            ff.resize(fst[ncf]);
+            std::vector<int> a(cst.begin(), cst.end() - 1), bi(ist.begin(), ist.end() - 1), bf(fst.begin(), fst.end() - 1);
+            for (int i = 0; i < nfc; ++i) cl[a[C.restrictMap[i]]++] = i;  // ascending fine cell
+            for (int f = 0; f < nff; ++f) {                               // ascending fine face
+                const int cf = C.faceRestrictMap[f];
+                if (cf < 0) {
+                    il[bi[-1 - cf]++] = f;
+                    continue;
+                }
+                // orientation (lduAgglomerateMatrix.cpp:50-66): the fine owner
+                // lies in the coarse owner, or the sides swap
+                const int ownerCell = C.restrictMap[fl[f]];
+                if (C.A.l[cf] == ownerCell)
+                    ff[bf[cf]++] = f;
+                else if (C.A.u[cf] == ownerCell)
+                    ff[bf[cf]++] = ~f;
+                else
+                    throw Error(UNAP_HIP_ERR_INTERNAL, "agglomeration: inconsistent coarse addressing");
+            }
+            AgglDevLevel& D = dev[k];
+            D.childStart.upload(cst, c.stream);
+            D.child.upload(cl, c.stream);
+            D.innerStart.upload(ist, c.stream);
+            D.inner.upload(il, c.stream);
+            D.cfStart.upload(fst, c.stream);
+            D.cfFine.upload(ff, c.stream);
+            c.sync();
+        }
+    }
+    fineD.upload(fine.d, c.stream);
+    fineUp.upload(fine.up, c.stream);
+    if (!fine.symm)
+        fineLo.upload(fine.lo, c.stream);
+    else
+        fineLo.release();
+    for (int k = 0; k < K; ++k) {
+        AgglDevLevel& D = dev[k];
+        const int nc = levels[k].nCoarse, ncf = levels[k].A.nFaces();
+        const double* fd = (k == 0) ? fineD.get() : dev[k - 1].d.get();
+        const double* fu = (k == 0) ? fineUp.get() : dev[k - 1].up.get();
+        const double* flo = fine.symm ? nullptr : ((k == 0) ? fineLo.get() : dev[k - 1].lo.get());
+        if (D.d.n != (size_t)nc) D.d.alloc(nc);
+        if (D.up.n != (size_t)std::max(ncf, 1)) D.up.alloc(std::max(ncf, 1));
+        if (fine.symm)
+            D.lo.release();
+        else if (D.lo.n != (size_t)std::max(ncf, 1))
+            D.lo.alloc(std::max(ncf, 1));
+        kGalerkinDiag<<<(nc + 255) / 256, 256, 0, c.stream>>>(nc, D.childStart.get(), D.child.get(),
+                                                              D.innerStart.get(), D.inner.get(), fd, fu, flo,
+                                                              D.d.get());
+        UH_LAUNCHED();
+        if (ncf > 0) {
+            kGalerkinFaces<<<(ncf + 255) / 256, 256, 0, c.stream>>>(ncf, D.cfStart.get(), D.cfFine.get(), fu, flo,
+                                                                    D.up.get(), fine.symm ? nullptr : D.lo.get());
+            UH_LAUNCHED();
+        }
+        levels[k].A.symm = fine.symm;
+    }
+    c.sync();
+    hostValid.assign(K, 0);
+    valuesGen = gen;
+}
+
+const HostLdu& AgglHost::hostLevel(int k) {
+    if (valuesGen < 0) throw Error(UNAP_HIP_ERR_INTERNAL, "coarse values requested before the Galerkin product");
+    if ((int)hostValid.size() != (int)levels.size()) hostValid.assign(levels.size(), 0);
+    if (!hostValid[k]) {
+        HostLdu& A = levels[k].A;
+        const AgglDevLevel& D = dev[k];
+        A.d.resize(A.n);
+        A.up.resize(A.nFaces());
+        A.lo.resize(A.symm ? 0 : A.nFaces());
+        UH_CHECK(hipMemcpy(A.d.data(), D.d.get(), sizeof(double) * A.n, hipMemcpyDeviceToHost));
+        if (A.nFaces() > 0) {
+            UH_CHECK(hipMemcpy(A.up.data(), D.up.get(), sizeof(double) * A.nFaces(), hipMemcpyDeviceToHost));
+            if (!A.symm)
+                UH_CHECK(hipMemcpy(A.lo.data(), D.lo.get(), sizeof(double) * A.nFaces(), hipMemcpyDeviceToHost));
+        }
+        hostValid[k] = 1;
+    }
+    return levels[k].A;
+}
+
+template <typename T>
+void Hierarchy<T>::refresh() {
+    if (valuesGen == fineObj->coeffGen) return;
+    Ctx& c = Ctx::get();
+    if (ag->valuesGen != fineObj->coeffGen) ag->galerkin(fineObj->host, fineObj->coeffGen);
+    for (int l = 0; l < L; ++l)
+        if (mats[l]) {
+            const AgglDevLevel& D = ag->dev[sel[l]];
+            mats[l]->fillOnDevice(D.d.get(), D.up.get(), D.lo.p ? D.lo.get() : nullptr, c.stream);
+        }
+    c.sync();
+    setupNumeric();
+    valuesGen = fineObj->coeffGen;
+}
+
 template <typename T>
 void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgParams& p) {
     Ctx& c = Ctx::get();
@@ -116,6 +244,7 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     const bool fp32 = sizeof(T) == 4;
     const bool fixed = prm.fixedCycle;
     if (fp32 && !fixed) throw Error(UNAP_HIP_ERR_ARG, "the FP32 hierarchy needs the fixed cycle");
+    if (ag->valuesGen != A.coeffGen) ag->galerkin(A.host, A.coeffGen);
     mats.clear();
     cs.clear();
     ch.clear();
@@ -141,8 +270,9 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
         f2c.back().upload(tr.fineToCoarse, c.stream);
         c.sync();
         if (!(fixed && l == L - 1)) {
+            const AgglDevLevel& D = ag->dev[sel[l]];
             mats[l].reset(new DevMatrix<T>);
-            mats[l]->build(lev(l).A, cl, c.stream);
+            mats[l]->buildOnDevice(cl, D.d.get(), D.up.get(), D.lo.p ? D.lo.get() : nullptr, c.stream);
         }
         work[l].ensure(lev(l).nCoarse);
     }
@@ -157,15 +287,24 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     firstCall.assign(L, 1);
     est.alloc(L);
     mgst.alloc(1);
+    setupNumeric();
+    valuesGen = A.coeffGen;
+}
 
-    const HostLdu& CA = lev(L - 1).A;
+// Coarsest-level data from the current Galerkin values: the reference
+// cycle's rD and LDS copy, or the fixed cycle's dense inverse and Chebyshev
+// estimates.
+template <typename T>
+void Hierarchy<T>::setupNumeric() {
+    Ctx& c = Ctx::get();
+    const bool fixed = prm.fixedCycle;
+    const HostLdu& CA = ag->hostLevel(sel[L - 1]);
     const Layout& CL = ag->layoutAt(order, sel[L - 1]);
     if (!fixed) {
         // reference path: PCG+DIC / PBiCGStab+DILU to relTol 1e-2 in one workgroup
         coarsest64.reset();
         DevMatrix<double>* cm = reinterpret_cast<DevMatrix<double>*>(mats[L - 1].get());
         if (!cm->sweepable) throw Error(-3, "coarsest level faces are not owner-sorted");
-        if (CA.symm == false) cm->buildValU(CA, CL, c.stream);
         std::vector<double> rdExt = CA.symm ? dicReciprocalD(CA) : diluReciprocalD(CA);
         std::vector<double> rdInt(CL.n);
         for (int i = 0; i < CL.n; ++i) rdInt[i] = rdExt[CL.perm[i]];
@@ -431,6 +570,8 @@ void applyPrecond64(PrecondObj& P, const double* r, double* w, Skip sk) {
         case UNAP_HIP_PRECOND_DIC:
         case UNAP_HIP_PRECOND_DILU:
             if (P.lu32) throw Error(-4, "FP32 DIC/DILU preconditioners need the FCG or PBiCGStab solver");
+            if (P.type == UNAP_HIP_PRECOND_DIC)  // upper on both sides once the storage is asymmetric
+                P.A->dev64(P.order).buildValU(P.A->host, P.A->layout(P.order), Ctx::get().stream);
             luSweeps<double, double>(P.A->dev64(P.order), P.type == UNAP_HIP_PRECOND_DIC, w, r, P.rD.get(), sk,
                                      P.rDpos.get());
             break;
@@ -798,7 +939,13 @@ int unap_hip_ldu_set_coeffs(unap_hip_ldu A, const double* diag, const double* up
         H.lo.clear();
         if (!H.symm) H.lo.assign(lower, lower + H.nFaces());
         Ctx& c = Ctx::get();
+        // New preconditioners agglomerate from the new weights; existing
+        // hierarchies keep their maps and redo their Galerkin levels at their
+        // next use (Hierarchy::refresh).  DIC/DILU/diagonal preconditioners
+        // keep the rD they computed at creation, as the reference's do
+        // (lduDICPrecond.cpp:4-13).
         A->o.agglCache.reset();
+        A->o.coeffGen += 1;
         for (LduView& v : A->o.views) {
             if (v.d64) v.d64->updateValues(H, *v.layout, c.stream);
             if (v.d32) v.d32->updateValues(H, *v.layout, c.stream);
@@ -931,6 +1078,8 @@ int unap_hip_precond_apply(unap_hip_precond P, const double* r, double* w, int w
         DevMatrix<double>& M = o.A->dev64(o.order);
         DBuf<double> ri(M.n), wi(M.n);
         toInternal(M, r, where, ri.get(), stageBuf());
+        if (o.amg32) o.amg32->refresh();
+        if (o.amg64) o.amg64->refresh();
         if (o.amg32) {
             DBuf<float> w32(M.n);
             o.amg32->initSmoothers();
@@ -1109,6 +1258,11 @@ int unap_hip_aggl_coarse_matrix(unap_hip_aggl G, int64_t l, int64_t* la, int64_t
                                 double* upper, int* symm) {
     return guard([&] {
         const CoarseLevel& C = level(G, l);
+        if (lower || diag || upper) {
+            // Galerkin values (on the GPU) of the matrix's current coefficients
+            if (G->ag->valuesGen != G->A->coeffGen) G->ag->galerkin(G->A->host, G->A->coeffGen);
+            G->ag->hostLevel((int)l);
+        }
         const HostLdu& A = C.A;
         const int nf = A.nFaces();
         for (int f = 0; f < nf; ++f) {
@@ -1344,7 +1498,7 @@ int unap_hip_solver_set_mg(unap_hip_solver S, const unap_hip_amg_params* amg, co
         need(S->o.mg.precision == 64, "the standalone MG solver runs in FP64");
         S->o.mgSet = true;
         S->o.mgH.reset();
-        S->o.mgFor = nullptr;
+        S->o.mgFor = 0;
         S->o.mgHasWeights = weights != nullptr;
         S->o.mgWeights.clear();
         S->o.mgWeightsPtr = weights;  // copied at the next solve, when the face count is known
@@ -1382,12 +1536,16 @@ int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const do
         }
         if (o.type == UNAP_HIP_SOLVER_MG && o.mgHasWeights)
             need((int64_t)o.mgWeights.size() == L.host.nFaces(), "MG weights were given for a matrix with another face count");
-        if (o.type == UNAP_HIP_SOLVER_MG && (!o.mgH || o.mgFor != &L)) {
+        if (o.type == UNAP_HIP_SOLVER_MG && (!o.mgH || o.mgFor != L.id)) {
             auto ag = agglFor(L, o.mgHasWeights ? o.mgWeights.data() : nullptr, true, o.mg.maxLevels, o.mg.nCoarsest);
             o.mgH.reset(new Hierarchy<double>);
             o.mgH->build(L, ag, o.mg);
-            o.mgFor = &L;
+            o.mgFor = L.id;
         }
+        // coefficients updated since the hierarchy last ran: new Galerkin levels
+        if (o.type == UNAP_HIP_SOLVER_MG) o.mgH->refresh();
+        if (o.P && o.P->amg64) o.P->amg64->refresh();
+        if (o.P && o.P->amg32) o.P->amg32->refresh();
         if (o.P && o.P->amg64) o.P->amg64->initSmoothers();
         hipEvent_t e0, e1;
         UH_CHECK(hipEventCreate(&e0));

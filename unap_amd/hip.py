@@ -273,6 +273,13 @@ class LduMatrix:
         return {"n": n.value, "nfaces": nf.value, "symmetric": bool(sy.value),
                 "fwd_levels": fl.value, "bwd_levels": bl.value}
 
+    def set_coeffs(self, diag, upper, lower=None):
+        """setMatrixCoeffients: new values, same addressing (lower None: symmetric)."""
+        self._d = np.ascontiguousarray(diag, np.float64)
+        self._up = np.ascontiguousarray(upper, np.float64)
+        self._lo = None if lower is None or lower is upper else np.ascontiguousarray(lower, np.float64)
+        _check(lib().unap_hip_ldu_set_coeffs(self.h, _p(self._d), _p(self._up), _p(self._lo)))
+
     def spmv(self, x, out=None):
         xp, where, keep = _vec(x)
         if where == HOST:
@@ -380,7 +387,9 @@ class Agglomeration:
         _check(lib().unap_hip_aggl_nlevels(self.h, ctypes.byref(n)))
         return n.value
 
-    def level(self, lev, fine_n, fine_nf):
+    def level(self, lev, fine_n, fine_nf, values=True):
+        """Maps and coarse addressing of one level; with values, its Galerkin
+        coefficients too (formed on the GPU)."""
         nc, nf = _i64(), _i64()
         L = lib()
         _check(L.unap_hip_aggl_level_size(self.h, lev, ctypes.byref(nc), ctypes.byref(nf)))
@@ -391,20 +400,23 @@ class Agglomeration:
         _check(L.unap_hip_aggl_face_restrict_addressing(self.h, lev, _p(fa, _i64p)))
         l = np.empty(nf, np.int64)
         u = np.empty(nf, np.int64)
-        lo = np.empty(nf)
-        up = np.empty(nf)
-        d = np.empty(nc)
+        lo = np.empty(nf) if values else None
+        up = np.empty(nf) if values else None
+        d = np.empty(nc) if values else None
         sy = ctypes.c_int()
         _check(L.unap_hip_aggl_coarse_matrix(self.h, lev, _p(l, _i64p), _p(u, _i64p), _p(lo), _p(d), _p(up),
                                              ctypes.byref(sy)))
-        return {"n": nc, "nfaces": nf, "restrict": ra, "face_restrict": fa, "l": l, "u": u, "lower": lo,
-                "diag": d, "upper": up, "symmetric": bool(sy.value)}
+        out = {"n": nc, "nfaces": nf, "restrict": ra, "face_restrict": fa, "l": l, "u": u,
+               "symmetric": bool(sy.value)}
+        if values:
+            out.update(lower=lo, diag=d, upper=up)
+        return out
 
-    def levels(self):
+    def levels(self, values=True):
         out = []
         n, nf = self.A.n, self.A._u.shape[0]
         for lev in range(self.nlevels()):
-            lv = self.level(lev, n, nf)
+            lv = self.level(lev, n, nf, values)
             out.append(lv)
             n, nf = lv["n"], lv["nfaces"]
         return out
