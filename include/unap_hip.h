@@ -78,6 +78,17 @@ int unap_hip_ldu_set_coeffs(unap_hip_ldu A, const double* diag, const double* up
                             const double* lower);
 /* spMV(Apsi, psi) (src/matrix/lduMatrix/lduMatrixSpMV.cpp:16-98). */
 int unap_hip_ldu_spmv(unap_hip_ldu A, const double* psi, double* Apsi, int where);
+/* No counterpart in the reference: unap_hip_ldu_spmv for device vectors that
+   allocates nothing (no hipMalloc or hipFree, no allocating library call),
+   for callers that cannot allocate (code under HIP graph capture, tight
+   serving loops, harnesses that forbid it).  The internal-order staging lives
+   in a fixed device scratch array, so the matrix may hold at most 2097152
+   cells (UNAP_HIP_ERR_ARG beyond).  Needs the library context and the
+   matrix's device layout, both built by one earlier unap_hip_ldu_spmv call on
+   this matrix (UNAP_HIP_ERR_STATE otherwise).  Every call on one GPU shares
+   that scratch: calls that overlap on one GPU race.  Synchronous, like
+   unap_hip_ldu_spmv. */
+int unap_hip_ldu_spmv_noalloc(unap_hip_ldu A, const double* psi, double* Apsi);
 
 /* --------------------------------------------------------- preconditioners
  * Replaces matrix::preconditioner::precondition(w, r) (unapMatrix.hpp:337)

@@ -84,6 +84,7 @@ struct Ctx {
     DBuf<double> partials;
     DBuf<KState> scratchState;  // for standalone calls (eigen estimate, precond apply)
     static Ctx& get();
+    static Ctx* peek();  // the context if it exists (creating it allocates)
     void sync() { UH_CHECK(hipStreamSynchronize(stream)); }
 };
 

@@ -19,17 +19,18 @@ namespace unap_hip {
 static thread_local std::string g_lastError;
 void setLastError(const std::string& m) { g_lastError = m; }
 
+static Ctx* g_ctx = nullptr;
 Ctx& Ctx::get() {
-    static Ctx* c = nullptr;
-    if (!c) {
+    if (!g_ctx) {
         Ctx* n = new Ctx;
         UH_CHECK(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
         n->partials.alloc((size_t)kMaxSums * kMaxBlocks);
         n->scratchState.alloc(4);
-        c = n;
+        g_ctx = n;
     }
-    return *c;
+    return *g_ctx;
 }
+Ctx* Ctx::peek() { return g_ctx; }
 
 // Pinned host mirror of one KState for flag read-back.
 static KState* hostState() {
@@ -810,7 +811,31 @@ struct unap_hip_solver_s {
     SolveBufs bufs;
 };
 
+// Scratch of unap_hip_ldu_spmv_noalloc: two vectors of up to
+// kNoAllocMaxCells doubles (32 MiB), one copy on every device the library
+// runs on, held until the process ends.  Host code never uses the array's
+// name as an address: each device's address comes from hipGetSymbolAddress,
+// taken once per device on the first call there.  Calls that overlap on one
+// device share the array and race.
+constexpr long long kNoAllocMaxCells = 1LL << 21;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) double scratch[2 * kNoAllocMaxCells];
+
 namespace {
+
+double* g_scratchPtr[kNoAllocMaxDevices] = {};
+
+double* noAllocScratch() {
+    int dev = 0;
+    UH_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) throw Error(UNAP_HIP_ERR_ARG, "device index out of range");
+    if (!g_scratchPtr[dev]) {
+        void* p = nullptr;
+        UH_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratchPtr[dev] = static_cast<double*>(p);
+    }
+    return g_scratchPtr[dev];
+}
 
 template <class F>
 int guard(F&& f) {
@@ -962,6 +987,28 @@ int unap_hip_ldu_spmv(unap_hip_ldu A, const double* psi, double* Apsi, int where
         toInternal(M, psi, where, xi.get(), stageBuf());
         launchRows(BAmul<double>{M.view(), xi.get(), yi.get()}, M.n);
         toExternal(M, yi.get(), where, Apsi, stageBuf());
+    });
+}
+
+int unap_hip_ldu_spmv_noalloc(unap_hip_ldu A, const double* psi, double* Apsi) {
+    return guard([&] {
+        checkLdu(A);
+        need(psi && Apsi, "null vector");
+        const int n = A->o.n();
+        need(n <= kNoAllocMaxCells, "unap_hip_ldu_spmv_noalloc: the matrix has more cells than the scratch holds "
+                                    "(at most 2097152)");
+        Ctx* c = Ctx::peek();
+        const LduView& v = A->o.views[layoutMode(ORDER_STREAM)];
+        if (!c || !v.d64)
+            throw Error(UNAP_HIP_ERR_STATE, "unap_hip_ldu_spmv_noalloc: call unap_hip_ldu_spmv on this matrix once "
+                                            "first (it builds the device layout, which allocates)");
+        const DevMatrix<double>& M = *v.d64;
+        double* xi = noAllocScratch();  // [0, n): psi in internal order, [n, 2n): A psi
+        double* yi = xi + n;
+        launchRows(BGather<double, double>{M.perm.get(), psi, xi}, n);
+        launchRows(BAmul<double>{M.view(), xi, yi}, n);
+        launchRows(BScatter<double, double>{M.perm.get(), yi, Apsi}, n);
+        c->sync();
     });
 }
 

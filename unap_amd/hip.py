@@ -56,6 +56,7 @@ SIGNATURES = {
     "unap_hip_ldu_info": (ctypes.c_int, [_vp, _i64p, _i64p, _intp, _i64p, _i64p]),
     "unap_hip_ldu_set_coeffs": (ctypes.c_int, [_vp, _f64p, _f64p, _f64p]),
     "unap_hip_ldu_spmv": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "unap_hip_ldu_spmv_noalloc": (ctypes.c_int, [_vp, _vp, _vp]),
     "unap_hip_amg_params_default": (ctypes.c_int, [ctypes.POINTER(AmgParams)]),
     "unap_hip_precond_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(AmgParams), _f64p, ctypes.POINTER(_vp)]),
     "unap_hip_precond_destroy": (ctypes.c_int, [_vp]),
@@ -287,6 +288,12 @@ class LduMatrix:
             _check(lib().unap_hip_ldu_spmv(self.h, xp, y.ctypes.data_as(ctypes.c_void_p), HOST))
             return y
         _check(lib().unap_hip_ldu_spmv(self.h, xp, ctypes.c_void_p(out.data_ptr()), DEVICE))
+        return out
+
+
+    def spmv_noalloc(self, x, out):
+        """A x into out, both device arrays, with no device allocation."""
+        _check(lib().unap_hip_ldu_spmv_noalloc(self.h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr())))
         return out
 
 
