@@ -120,7 +120,7 @@ void launchReduce(const Body& b, int n, const int* order, const Fin& fin, Skip s
     int nb = L.grid;
     if (c.serialReductions) {
         // reference order: one thread sums the terms serially
-        kSerialSum<Body, NS><<<1, 64, 0, c.stream>>>(b, order, n, c.partials.get(), sk);
+        kSerialSum<Body, NS><<<1, kSerialChunk, 0, c.stream>>>(b, order, n, c.partials.get(), sk);
         UH_LAUNCHED();
         nb = 1;
     }

@@ -240,26 +240,53 @@ __global__ void __launch_bounds__(kBlock) kRows(Body body, int n, Skip skip) {
     UH_ROWS(r, n) body(r);
 }
 
-// Reference-order reduction: one thread sums body.terms(i) over the caller's
-// cell order (internal id iperm[e] for e = 0, 1, ...), exactly the serial
+// Reference-order reduction: body.terms(i) summed over the caller's cell
+// order (internal id iperm[e] for e = 0, 1, ...), exactly the serial
 // left-to-right loops of the reference (unapVectorOperations.cpp:21-30).
-// Used when bit-identical residual histories are wanted; O(n) sequential.
+// Used when bit-identical residual histories are wanted.  One block of
+// kSerialChunk threads: each fetches the terms of one element of a chunk
+// into LDS, then thread 0 adds the chunk in element order while the block's
+// loads of the next chunk are already in flight (each thread issues them
+// before the barrier that ends the chunk).  The additions are the serial
+// loop's; only the loads run in parallel.
+constexpr int kSerialChunk = 256;
 template <class Body, int NS>
-__global__ void kSerialSum(Body body, const int* __restrict__ iperm, int n, double* __restrict__ partials,
-                           Skip skip) {
+__global__ void __launch_bounds__(kSerialChunk) kSerialSum(Body body, const int* __restrict__ iperm, int n,
+                                                           double* __restrict__ partials, Skip skip) {
     if (skip.on()) return;
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double acc[NS];
+    __shared__ double sm[2][NS][kSerialChunk];
+    const int t = threadIdx.x;
+    double acc[NS], v[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) acc[s] = 0.0;
-    for (int e = 0; e < n; ++e) {
-        double t[NS];
-        body.terms(iperm[e], t);
-#pragma unroll
-        for (int s = 0; s < NS; ++s) acc[s] = acc[s] + t[s];
+    for (int s = 0; s < NS; ++s) {
+        acc[s] = 0.0;
+        v[s] = 0.0;
     }
+    if (t < n) body.terms(iperm[t], v);
 #pragma unroll
-    for (int s = 0; s < NS; ++s) partials[s * kMaxBlocks] = acc[s];
+    for (int s = 0; s < NS; ++s) sm[0][s][t] = v[s];
+    __syncthreads();
+    for (int e0 = 0, buf = 0; e0 < n; e0 += kSerialChunk, buf ^= 1) {
+        // next chunk's terms: loads issued now, stored after the serial adds
+        const int e1 = e0 + kSerialChunk + t;
+        if (e1 < n) body.terms(iperm[e1], v);
+        if (t == 0) {
+            const int m = min(kSerialChunk, n - e0);
+            for (int j = 0; j < m; ++j) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s) acc[s] = acc[s] + sm[buf][s][j];
+            }
+        }
+        if (e1 < n) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) sm[buf ^ 1][s][t] = v[s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) partials[s * kMaxBlocks] = acc[s];
+    }
 }
 
 // One block: total of each partials slot in index order, then fin(totals).
