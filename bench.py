@@ -266,7 +266,7 @@ def config_legs(N, with_cpu, which=("c1", "c2", "c3"), reps=3, with_mixed=True):
         "c2": ("FCG", "AMG", {"precision": 32, "smoother": 1, "n_cells_coarsest": 500, "n_vcycles": 1,
                               "cycle": hip.CYCLE_FIXED},
                "FCG + FP32 AMG (Gauss-Seidel, fixed cycle, dense FP64 coarsest)"),
-        "c3": ("PBiCGStab", "DILU", {"precision": 32}, "PBiCGStab + FP32 DILU"),
+        "c3": ("FBiCGStab", "DILU", {"precision": 32}, "flexible BiCGStab + FP32 DILU"),
     }
     # the FP64 solution the errors are measured against
     ref_legs = {

@@ -209,6 +209,12 @@ int unap_hip_aggl_prolong_field(unap_hip_aggl G, int64_t level, const double* co
 #define UNAP_HIP_SOLVER_PBICGSTAB 2
 #define UNAP_HIP_SOLVER_MG 3
 #define UNAP_HIP_SOLVER_FCG 4
+/* Flexible BiCGStab (no reference counterpart): PBiCGStab's loop
+   (PBiCGStab.cpp:45-459), right-preconditioned so that a variable
+   preconditioner keeps the x update consistent, with the shadow residual
+   restarted (r0 = r) once |r0.r| < 1e-6 |r0| |r|.  Identical to PBiCGStab
+   while no restart triggers; meant for FP32 DIC/DILU preconditioners. */
+#define UNAP_HIP_SOLVER_FBICGSTAB 5
 
 /* solverPerformance (unapMatrix.hpp:43-174) */
 typedef struct {

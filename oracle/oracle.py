@@ -18,7 +18,7 @@ _f64p = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
 
 PRECOND = {"none": 0, "diag": 1, "DIC": 2, "DILU": 3, "AMG": 4, "GS1": 5}  # GS1: oracle only
-SOLVER = {"PCG": 1, "PBiCGStab": 2, "FCG": 4}
+SOLVER = {"PCG": 1, "PBiCGStab": 2, "FCG": 4, "FBiCGStab": 5}
 SMOOTHER = {"GS": 1, "Cheby": 2}
 
 

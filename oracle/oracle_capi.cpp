@@ -166,6 +166,8 @@ int uo_krylov_solve(void* Av, int solver, int precond, const int64_t* mg, int64_
         S.reset(new PBiCGStab(*P));
     else if (solver == 4)
         S.reset(new FCG(*P));
+    else if (solver == 5)
+        S.reset(new FBiCGStab(*P));
     else
         return -2;
     S->maxIter = maxIter;
@@ -277,6 +279,8 @@ void* uo_solver_create(void* Av, int solver, int precond, const int64_t* mg) {
         b->S.reset(new PBiCGStab(*b->P));
     else if (solver == 4)
         b->S.reset(new FCG(*b->P));
+    else if (solver == 5)
+        b->S.reset(new FBiCGStab(*b->P));
     else
         return nullptr;
     return b.release();

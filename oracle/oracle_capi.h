@@ -60,7 +60,7 @@ void OFN(aggl_coarse_matrix)(void* G, int64_t lev, int64_t* l, int64_t* u, doubl
                              double* diag, double* upper, int64_t* symm);
 void OFN(aggl_destroy)(void* G);
 
-/* Krylov solve.  solver: 1 PCG, 2 PBiCGStab, 4 FCG (oracle only).
+/* Krylov solve.  solver: 1 PCG, 2 PBiCGStab, 4 FCG, 5 FBiCGStab (oracle only).
    precond: 0 identity, 1 diag, 2 DIC, 3 DILU, 4 AMG (mg[] used), 5 one
    Gauss-Seidel sweep (oracle only).
    mg[] = {smootherType, nCoarsest, nVcycles, nPre, nPost, nFinest} (may be NULL

@@ -566,11 +566,21 @@ Perf PBiCGStab::solve(double* x, const Ldu& A, const double* b) const {
     if (history) history->assign(1, perf.initial);
     std::vector<double> Ay(n, 0.0), s(n, 0.0), z(n, 0.0), t(n, 0.0);
     double rho = 0, alpha = 0, omega = 0;
+    // flexible variant (shadowRestart > 0): |r0| and the iterations since r0
+    // was (re)set
+    double r0norm = perf.initial - SMALL_;
+    label k = 0;
     do {
         const double rhoOld = rho;
         rho = dot(r0.data(), r.data(), n);
+        if (shadowRestart > 0 && k > 0 && std::fabs(rho) < shadowRestart * r0norm * perf.final_) {
+            for (label i = 0; i < n; ++i) r0[i] = r[i];
+            rho = dot(r.data(), r.data(), n);
+            r0norm = std::sqrt(rho);
+            k = 0;
+        }
         if (checkSingularity(perf, std::fabs(rho))) break;
-        if (perf.nIter == 0) {
+        if ((shadowRestart > 0 ? k : perf.nIter) == 0) {
             for (label i = 0; i < n; ++i) p[i] = r[i];
         } else {
             const double beta = (rho / rhoOld) * (alpha / omega);
@@ -599,6 +609,7 @@ Perf PBiCGStab::solve(double* x, const Ldu& A, const double* b) const {
         }
         perf.final_ = normFactor(r.data(), n);
         if (history) history->push_back(perf.final_);
+        ++k;
     } while (++perf.nIter < maxIter && !checkConvergence(perf, tol, relTol, perf.nIter, minIter));
     return perf;
 }

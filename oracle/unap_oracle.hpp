@@ -162,8 +162,16 @@ struct PCG : Solver {
 
 struct PBiCGStab : Solver {
     const Precond* M;
+    // 0: the reference's PBiCGStab.  > 0: the flexible variant (no reference
+    // counterpart), which restarts the shadow residual, r0 = r, once
+    // |r0.r| < shadowRestart |r0| |r| (then p = r as in a first iteration).
+    double shadowRestart = 0;
     explicit PBiCGStab(const Precond& m) : M(&m) {}
     Perf solve(double* x, const Ldu& A, const double* b) const override;
+};
+
+struct FBiCGStab : PBiCGStab {
+    explicit FBiCGStab(const Precond& m) : PBiCGStab(m) { shadowRestart = 1.0e-6; }
 };
 
 // Flexible CG (Notay 2000 FCG(1): each direction A-orthogonalised against the

@@ -1,5 +1,5 @@
 """FP32 DIC / DILU preconditioners inside FP64 Krylov solvers (FCG for DIC,
-PBiCGStab for DILU): the sweeps, their coefficients and rD in FP32, the
+PBiCGStab or the flexible BiCGStab for DILU): the sweeps, their coefficients and rD in FP32, the
 residual, dot products, norms and convergence test in FP64.  There is no
 counterpart in the reference, so the checks are against the FP64 oracle:
 the FP32 apply agrees with the FP64 apply to FP32 accuracy, each solve meets
@@ -47,13 +47,14 @@ def test_fcg_dic32(oracle, c, N, tol):
     _check(oracle, s, "FCG", "DIC", tol, x, p)
 
 
+@pytest.mark.parametrize("solver", ["PBiCGStab", "FBiCGStab"])
 @pytest.mark.parametrize("c,N", [(3, 16), (3, 24), (2, 16)])
-def test_bicgstab_dilu32(oracle, c, N):
+def test_bicgstab_dilu32(oracle, solver, c, N):
     s = meshgen.config_system(c, N)
     A = hip.LduMatrix.from_system(s)
-    S = hip.Solver("PBiCGStab", hip.Precond(A, "DILU", precision=32), rel_tol=1e-8, max_iter=2000)
+    S = hip.Solver(solver, hip.Precond(A, "DILU", precision=32), rel_tol=1e-8, max_iter=2000)
     x, p = S.solve(A, s.b)
-    _check(oracle, s, "PBiCGStab", "DILU", 1e-8, x, p)
+    _check(oracle, s, solver, "DILU", 1e-8, x, p)
 
 
 def test_fp32_lu_needs_flexible_solver():

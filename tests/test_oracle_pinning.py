@@ -451,3 +451,80 @@ def test_bicgstab_history_sensitivity(oracle):
     assert d[:21].max() <= 1e-8
     assert d[: int(0.3 * n_it)].max() > 1e-6
     assert np.linalg.norm(xt - xs) / np.linalg.norm(xs) < 1e-6
+
+
+def _fbicg_emulation(M, P, s, max_iter=2000, tol=1e-8, thr=1e-6):
+    """PBiCGStab with the shadow-residual restart, serial dots and the
+    oracle's operation order.  Returns x, the history, and for every restart
+    the iteration index and the iterate x at that point."""
+    n = s.n
+    x = np.zeros(n)
+    r = s.b - M.spmv(x)
+    r0 = r.copy()
+    init = np.sqrt(_serial_dot(r, r)) + 1e-37
+    hist, fin, r0n = [init], init, init - 1e-37
+    p = np.zeros(n)
+    Ay = np.zeros(n)
+    rho = alpha = omega = 0.0
+    it = k = 0
+    restarts = []
+    while True:
+        rho_old, rho = rho, _serial_dot(r0, r)
+        if k > 0 and abs(rho) < thr * r0n * fin:
+            r0 = r.copy()
+            rho = _serial_dot(r, r)
+            r0n = np.sqrt(rho)
+            k = 0
+            restarts.append((it, x.copy()))
+        p = r.copy() if k == 0 else r + (rho / rho_old) * (alpha / omega) * (p - omega * Ay)
+        y = P.apply(p)
+        Ay = M.spmv(y)
+        alpha = rho / _serial_dot(r0, Ay)
+        sv = r - alpha * Ay
+        fs = np.sqrt(_serial_dot(sv, sv)) + 1e-37
+        if fs <= tol * init:
+            hist.append(fs)
+            return x + alpha * y, np.array(hist), restarts
+        z = P.apply(sv)
+        t = M.spmv(z)
+        tt = _serial_dot(t, t)
+        omega = _serial_dot(t, sv) / tt
+        x = x + (alpha * y + omega * z)
+        r = sv - omega * t
+        fin = np.sqrt(_serial_dot(r, r)) + 1e-37
+        hist.append(fin)
+        it += 1
+        k += 1
+        if not it < max_iter or fin <= tol * init:
+            return x, np.array(hist), restarts
+
+
+@pytest.mark.parametrize("c,N,pre", [(3, 16, "GS1"), (2, 12, "DILU"), (4, 12, "DILU")])
+def test_fbicgstab_restart(oracle, c, N, pre):
+    """Flexible BiCGStab has no reference counterpart.  Pinned three ways:
+    the oracle reproduces a numpy emulation bit for bit through its shadow
+    restarts; a restart at iteration k behaves as the reference's PBiCGStab
+    started afresh from x_k (r0 = r, p = r), so the residuals that follow
+    track that solve's; and the solution meets the tolerance on the host."""
+    s = meshgen.config_system(c, N)
+    M = oracle.matrix(s)
+    P = M.precond_obj(pre)
+    xe, he, restarts = _fbicg_emulation(M, P, s)
+    assert restarts, "the case must restart"
+    xo, po = M.solve("FBiCGStab", pre, s.b, rel_tol=1e-8, max_iter=2000)
+    assert np.array_equal(po["history"], he) and np.array_equal(xo, xe)
+    assert po["converged"]
+    assert np.linalg.norm(s.b - M.spmv(xo)) <= 1.1e-8 * np.linalg.norm(s.b)
+    k, xk = restarts[0]
+    _, pf = M.solve("PBiCGStab", pre, s.b, x0=xk, rel_tol=1e-30, max_iter=4)
+    # fresh solve: its initial residual is the true b - A x_k, the flexible
+    # one continues from the recursively updated r_k (equal to ~1e-10)
+    assert np.allclose(pf["history"], he[k:k + 5], rtol=1e-7, atol=0)
+
+
+def test_fbicgstab_without_restart_is_pbicgstab(oracle, reference):
+    s = meshgen.config_system(3, 10)
+    xf, pf = oracle.matrix(s).solve("FBiCGStab", "DILU", s.b, rel_tol=1e-8, max_iter=500)
+    xr, pr = reference.matrix(s).solve("PBiCGStab", "DILU", s.b, rel_tol=1e-8, max_iter=500)
+    assert pf["nIterations"] == pr["nIterations"]
+    assert np.array_equal(pf["history"], pr["history"]) and np.array_equal(xf, xr)

@@ -50,6 +50,10 @@ struct KState {
     long long maxIter, minIter, nIter;
     int converged, singular, done, stop;
     long long exitIt;
+    // flexible BiCGStab: shadow-residual restart state
+    double r0norm;
+    long long kShadow;       // iterations since the shadow residual was (re)set
+    int restart, noRestart;  // this iteration restarts / does not
     double* hist;
     long long histCap;
     double alphas[32], betas[32];
@@ -334,6 +338,10 @@ static __global__ void kStateInit(KState* st, double tol, double relTol, long lo
     st->nIter = 0;
     st->converged = st->singular = st->done = st->stop = 0;
     st->exitIt = -1;
+    st->r0norm = 0.0;
+    st->kShadow = 0;
+    st->restart = 0;
+    st->noRestart = 1;
     st->hist = hist;
     st->histCap = histCap;
 }
@@ -413,6 +421,52 @@ struct FinBicgS {  // PBiCGStab.cpp:276-348: converged on ||s|| -> early exit
 struct FinBicgOmega {  // PBiCGStab.cpp:357-386: omega = (t.s) / (t.t)
     KState* st;
     __device__ void operator()(double (&t)[2]) const { st->omega = t[1] / t[0]; }
+};
+
+// ---------------------------------------------------- flexible BiCGStab steps
+// PBiCGStab's loop (PBiCGStab.cpp:45-459) with a variable preconditioner
+// (Vogel 2007: right preconditioning keeps x = x + alpha y + omega z
+// consistent whatever M^-1 is at each application) and a restart of the
+// shadow residual, r0 = r, once |r0.r| < kShadowRestart |r0| |r|: an FP32
+// preconditioner is not an exactly linear operator, and the shadow residual
+// loses bi-orthogonality with it.
+constexpr double kShadowRestart = 1.0e-6;
+struct FinFbicgRho {
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        if (st->nIter == 0) st->r0norm = st->initRes - kSmallD;  // r0 = r at the start
+        const bool restart = st->kShadow > 0 && fabs(t[0]) < kShadowRestart * st->r0norm * st->finalRes;
+        st->restart = restart ? 1 : 0;
+        st->noRestart = restart ? 0 : 1;
+        if (restart) return;  // rho comes from the restart's r.r
+        st->rhoOld = st->rho;
+        st->rho = t[0];
+        st->singular = (fabs(st->rho) > kSmallD) ? 0 : 1;
+        if (st->singular) {
+            st->done = 1;
+            return;
+        }
+        if (st->kShadow != 0) st->beta = (st->rho / st->rhoOld) * (st->alpha / st->omega);
+    }
+};
+struct FinFbicgRestart {  // r0 = r: rho = r.r, |r0| = sqrt(rho), directions restart
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        st->rhoOld = st->rho;
+        st->rho = t[0];
+        st->r0norm = sqrt(t[0]);
+        st->kShadow = 0;
+        st->noRestart = 1;
+        st->singular = (fabs(st->rho) > kSmallD) ? 0 : 1;
+        if (st->singular) st->done = 1;
+    }
+};
+struct FinFbicgOmega {
+    KState* st;
+    __device__ void operator()(double (&t)[2]) const {
+        st->omega = t[1] / t[0];
+        st->kShadow += 1;
+    }
 };
 
 // ---------------------------------------------------------------- FCG steps

@@ -151,6 +151,30 @@ struct BBicgP {  // PBiCGStab.cpp:131-175
             p[i] = r[i] + st->beta * (p[i] - st->omega * Ay[i]);
     }
 };
+// flexible BiCGStab: p = r at the start and after a shadow restart
+struct BFbicgP {
+    const double* r;
+    const double* Ay;
+    double* p;
+    const KState* st;
+    __device__ void operator()(int i) const {
+        if (st->kShadow == 0)
+            p[i] = r[i];
+        else
+            p[i] = r[i] + st->beta * (p[i] - st->omega * Ay[i]);
+    }
+};
+// shadow restart: r0 = r and r.r
+struct BShadowRestart {
+    const double* r;
+    double* r0;
+    __device__ void operator()(int i, double (&a)[1]) const {
+        const double ri = r[i];
+        r0[i] = ri;
+        a[0] += ri * ri;
+    }
+    __device__ void terms(int i, double (&t)[1]) const { t[0] = r[i] * r[i]; }
+};
 struct BAmulDotR0 {  // AyA = A yA; rA0.AyA (PBiCGStab.cpp:248-254)
     DMat<double> A;
     const double* y;

@@ -680,7 +680,8 @@ struct Driver {
         }
     }
 
-    void bicg() {
+    // flexible: the shadow-residual restart of FinFbicgRho (no reference counterpart)
+    void bicg(bool flexible = false) {
         double *x = B.x.get(), *b = B.b.get();
         double *r = B.v[0].get(), *r0 = B.v[1].get(), *p = B.v[2].get(), *y = B.v[3].get();
         double *Ay = B.v[4].get(), *s = B.v[5].get(), *z = B.v[6].get(), *t = B.v[7].get();
@@ -688,9 +689,16 @@ struct Driver {
         launchRows(BFill<double>{Ay, 0.0}, n);
         launchRows(BFill<double>{p, 0.0}, n);
         const Skip sk{&st->done, nullptr};
+        const Skip skRestart{&st->done, &st->noRestart};
         iterate([&](long long it) {
-            launchReduce<1>(BDot{r0, r}, n, M.iperm.get(), FinBicgRho{st}, sk);
-            launchRows(BBicgP{r, Ay, p, st}, n, sk);
+            if (flexible) {
+                launchReduce<1>(BDot{r0, r}, n, M.iperm.get(), FinFbicgRho{st}, sk);
+                launchReduce<1>(BShadowRestart{r, r0}, n, M.iperm.get(), FinFbicgRestart{st}, skRestart);
+                launchRows(BFbicgP{r, Ay, p, st}, n, sk);
+            } else {
+                launchReduce<1>(BDot{r0, r}, n, M.iperm.get(), FinBicgRho{st}, sk);
+                launchRows(BBicgP{r, Ay, p, st}, n, sk);
+            }
             precondInto(p, y, sk);
             launchReduce<1>(BAmulDotR0{M.view(), y, r0, Ay}, n, M.iperm.get(), FinBicgAlpha{st}, sk);
             launchReduce<1>(BBicgS{r, Ay, s, st}, n, M.iperm.get(), FinBicgS{st, it}, sk);
@@ -700,7 +708,10 @@ struct Driver {
                 UH_LAUNCHED();
             }
             precondInto(s, z, sk);
-            launchReduce<2>(BAmulTT{M.view(), z, s, t}, n, M.iperm.get(), FinBicgOmega{st}, sk);
+            if (flexible)
+                launchReduce<2>(BAmulTT{M.view(), z, s, t}, n, M.iperm.get(), FinFbicgOmega{st}, sk);
+            else
+                launchReduce<2>(BAmulTT{M.view(), z, s, t}, n, M.iperm.get(), FinBicgOmega{st}, sk);
             launchReduce<1>(BBicgXR{x, r, y, z, s, t, st}, n, M.iperm.get(), FinResNorm{st}, sk);
         });
     }
@@ -1509,7 +1520,7 @@ int unap_hip_profile_kernel(unap_hip_ldu A, int kernel, int precision, int64_t r
 int unap_hip_solver_create(int type, unap_hip_solver* out) {
     return guard([&] {
         need(out != nullptr, "out is null");
-        need(type >= 1 && type <= 4, "unknown solver type");
+        need(type >= 1 && type <= 5, "unknown solver type");
         auto* s = new unap_hip_solver_s;
         s->o.type = type;
         *out = s;
@@ -1565,8 +1576,9 @@ int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const do
         LduObj& L = A->o;
         need(!o.P || o.P->A == &L, "preconditioner was built for another matrix");
         need(!(o.P && o.P->amg32 && o.type != UNAP_HIP_SOLVER_FCG), "the FP32 AMG preconditioner needs FCG");
-        need(!(o.P && o.P->lu32 && o.type != UNAP_HIP_SOLVER_FCG && o.type != UNAP_HIP_SOLVER_PBICGSTAB),
-             "FP32 DIC/DILU preconditioners need FCG or PBiCGStab");
+        need(!(o.P && o.P->lu32 && o.type != UNAP_HIP_SOLVER_FCG && o.type != UNAP_HIP_SOLVER_PBICGSTAB &&
+               o.type != UNAP_HIP_SOLVER_FBICGSTAB),
+             "FP32 DIC/DILU preconditioners need FCG, PBiCGStab or FBiCGStab");
         need(o.type != UNAP_HIP_SOLVER_MG || o.mgSet, "MG solver needs unap_hip_solver_set_mg first");
         Ctx& c = Ctx::get();
         if (where == UNAP_HIP_DEVICE) UH_CHECK(hipDeviceSynchronize());
@@ -1607,6 +1619,7 @@ int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const do
         switch (o.type) {
             case UNAP_HIP_SOLVER_PCG: d.pcg(); break;
             case UNAP_HIP_SOLVER_PBICGSTAB: d.bicg(); break;
+            case UNAP_HIP_SOLVER_FBICGSTAB: d.bicg(true); break;
             case UNAP_HIP_SOLVER_FCG: d.fcg(); break;
             case UNAP_HIP_SOLVER_MG: d.mg(*o.mgH); break;
         }

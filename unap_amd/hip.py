@@ -18,7 +18,7 @@ HOST, DEVICE = 0, 1
 CYCLE_AUTO, CYCLE_REFERENCE, CYCLE_FIXED = -1, 0, 1
 PRECOND = {"none": 0, "diag": 1, "DIC": 2, "DILU": 3, "AMG": 4}
 SMOOTHER = {"GS": 1, "Cheby": 2}
-SOLVER = {"PCG": 1, "PBiCGStab": 2, "MG": 3, "FCG": 4}
+SOLVER = {"PCG": 1, "PBiCGStab": 2, "MG": 3, "FCG": 4, "FBiCGStab": 5}
 
 _i64 = ctypes.c_int64
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -444,7 +444,7 @@ class Agglomeration:
 
 
 class Solver:
-    """matrix::solver: PCG / PBiCGStab / MG / FCG with solve(x, A, b)."""
+    """matrix::solver: PCG / PBiCGStab / MG / FCG / FBiCGStab with solve(x, A, b)."""
 
     def __init__(self, kind, precond=None, max_iter=1000, min_iter=0, rel_tol=1e-6, tol=0.0, mg=None,
                  mg_weights=None):
