@@ -76,6 +76,16 @@ int unap_hip_ldu_info(unap_hip_ldu A, int64_t* n_cells, int64_t* n_faces, int* s
    same addressing; lower == upper or NULL keeps one coefficient array. */
 int unap_hip_ldu_set_coeffs(unap_hip_ldu A, const double* diag, const double* upper,
                             const double* lower);
+/* setMatrixTopology(upperAddr, lowerAddr) (lduMatrix.cpp:566-597): new
+   addressing, same cell count.  As there, the coefficients are not touched:
+   with the same face count they stay (now indexed by the new faces); with
+   another face count the matrix refuses every use (UNAP_HIP_ERR_STATE) until
+   unap_hip_ldu_set_coeffs gives values for the new faces.  Device layouts and
+   cached agglomerations are rebuilt on next use; preconditioners built on the
+   earlier addressing are refused (UNAP_HIP_ERR_STATE) where the reference's
+   would run on stale data. */
+int unap_hip_ldu_set_topology(unap_hip_ldu A, int64_t n_faces, const int64_t* lower_addr,
+                              const int64_t* upper_addr);
 /* spMV(Apsi, psi) (src/matrix/lduMatrix/lduMatrixSpMV.cpp:16-98). */
 int unap_hip_ldu_spmv(unap_hip_ldu A, const double* psi, double* Apsi, int where);
 /* No counterpart in the reference: unap_hip_ldu_spmv for device vectors that

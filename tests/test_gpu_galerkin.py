@@ -145,3 +145,34 @@ def test_set_coeffs_symmetric_to_asymmetric_dic(oracle):
     M.set_coeffs(d, up, lo)
     assert np.array_equal(P.apply(r), Po.apply(r))
     assert np.array_equal(A.spmv(r), M.spmv(r))
+
+
+def test_set_topology(oracle, reference_order):
+    """setMatrixTopology: a new addressing with the same face count keeps the
+    coefficient arrays (now read through the new faces); another face count
+    needs new coefficients first.  Amul and a PCG+DIC solve then match the
+    oracle on the resulting matrix; a preconditioner built on the old
+    addressing is refused."""
+    s = meshgen.config_system(2, 10)
+    t = meshgen.config_system(2, 10, seed=99)  # same box, another relabelling
+    A = hip.LduMatrix.from_system(s)
+    P_old = hip.Precond(A, "DIC")
+    A.set_topology(t.l, t.u)
+    st = meshgen.LduSystem(s.n, t.l, t.u, None, s.diag, s.upper, s.b, True)
+    x = np.random.default_rng(3).uniform(-1, 1, s.n)
+    assert np.array_equal(A.spmv(x), oracle.matrix(st).spmv(x))
+    with pytest.raises(hip.UnapHipError) as e:
+        hip.Solver("PCG", P_old, rel_tol=1e-8).solve(A, s.b)
+    assert e.value.code == -4
+    xg, pg = hip.Solver("PCG", hip.Precond(A, "DIC"), rel_tol=1e-8, max_iter=500).solve(A, s.b)
+    xo, po = oracle.matrix(st).solve("PCG", "DIC", s.b, rel_tol=1e-8, max_iter=500)
+    assert pg["nIterations"] == po["nIterations"] and np.array_equal(xg, xo)
+    # another face count: a smaller box's faces on the first cells
+    u = meshgen.config_system(2, 8)
+    A.set_topology(u.l, u.u)
+    with pytest.raises(hip.UnapHipError):
+        A.spmv(x)
+    d = np.concatenate([u.diag, np.ones(s.n - u.n)])
+    A.set_coeffs(d, u.upper)
+    su = meshgen.LduSystem(s.n, u.l, u.u, None, d, u.upper, s.b, True)
+    assert np.array_equal(A.spmv(x), oracle.matrix(su).spmv(x))

@@ -97,3 +97,43 @@ def test_agglomeration_matches_reference(reference, c, N, seed):
     for g, r in zip(got, ref):
         for k in ("restrict", "face_restrict", "l", "u"):
             assert np.array_equal(g[k], r[k]), k
+
+
+def test_set_topology_host_side():
+    # setMatrixTopology (lduMatrix.cpp:566-597): new addressing, same cells.
+    # Host-side effects only (no GPU call): the level counts follow the new
+    # faces, and another face count leaves the matrix refusing use until its
+    # coefficients are set.
+    s = meshgen.config_system(2, 6)
+    A = hip.LduMatrix.from_system(s)
+    chain_l = np.arange(s.n - 1, dtype=np.int64)
+    A.set_topology(chain_l, chain_l + 1)
+    info = A.info()
+    assert info["nfaces"] == s.n - 1 and info["fwd_levels"] == s.n and info["bwd_levels"] == s.n
+    with pytest.raises(hip.UnapHipError) as e:
+        A.spmv(np.ones(s.n))
+    assert e.value.code == -4
+    with pytest.raises(hip.UnapHipError) as e:
+        A.set_topology([0, 1], [1, s.n])
+    assert e.value.code == -1
+
+
+def test_reference_binding_header_compiles():
+    """unap_amd/integration/hipSolvers.hpp (the binding a maintainer adds to the
+    reference tree) compiles against the reference's own headers, with the
+    reference's build flags (its SConstruct: -DSCALAR_FLOAT64 -DLABEL_INT64)."""
+    import glob
+    import subprocess
+    ref = "/root/reference/src"
+    if not os.path.isdir(ref):
+        pytest.skip("the reference sources are not here")
+    inc = []
+    for d, _, _ in os.walk(ref):
+        if "swArrays" not in d and "swAthread" not in d:
+            inc += ["-I", d]
+    cmd = ["g++", "-std=c++11", "-fsyntax-only", "-DSCALAR_FLOAT64", "-DLABEL_INT64", "-w", *inc,
+           "-I", "/opt/conda/include", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(ROOT, "unap_amd", "integration"),
+           os.path.join(ROOT, "tests", "cpp", "ref_binding_example.cpp")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]

@@ -41,6 +41,10 @@ struct LduObj {
     // bumped by every coefficient update: hierarchies built on this matrix
     // recompute their Galerkin levels when it moves
     long long coeffGen = 0;
+    // bumped by every topology update: objects built on the old addressing
+    // refuse to run
+    long long topoGen = 0;
+    bool coeffsPending = false;  // a topology of another face count awaits its coefficients
     // process-unique id (caches keyed on a matrix never confuse two matrices
     // that lived at one address)
     const unsigned long long id = nextId();
@@ -368,6 +372,7 @@ struct PrecondObj {
     int type = 0;
     int order = ORDER_STREAM;  // cell order of the vectors it works on
     LduObj* A = nullptr;
+    long long topoGen = 0;           // topology generation of A at creation
     DBuf<double> rD;                 // internal order
     std::vector<double> rDext;       // external order (rD())
     std::unique_ptr<Hierarchy<double>> amg64;
@@ -402,6 +407,7 @@ struct SolverObj {
     const double* mgWeightsPtr = nullptr;  // caller-owned, read at the next solve
     std::unique_ptr<Hierarchy<double>> mgH;
     unsigned long long mgFor = 0;  // id of the matrix mgH was built for
+    long long mgTopoGen = -1;      // and its topology generation
     // results
     std::vector<double> history;
     double lastMs = 0;

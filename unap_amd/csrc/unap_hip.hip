@@ -900,6 +900,13 @@ DBuf<double>& stageBuf() {
 }
 
 void checkLdu(unap_hip_ldu A) { need(A != nullptr, "null matrix handle"); }
+// A matrix whose topology changed to another face count has no valid
+// coefficients until unap_hip_ldu_set_coeffs.
+void checkLduReady(unap_hip_ldu A) {
+    checkLdu(A);
+    if (A->o.coeffsPending)
+        throw Error(UNAP_HIP_ERR_STATE, "the matrix's topology changed its face count: set its coefficients first");
+}
 
 }  // namespace
 
@@ -969,6 +976,7 @@ int unap_hip_ldu_set_coeffs(unap_hip_ldu A, const double* diag, const double* up
         checkLdu(A);
         HostLdu& H = A->o.host;
         need(diag && (H.nFaces() == 0 || upper), "null array");
+        A->o.coeffsPending = false;
         H.d.assign(diag, diag + H.n);
         H.up.assign(upper, upper + H.nFaces());
         H.symm = (lower == nullptr || lower == upper);
@@ -989,9 +997,42 @@ int unap_hip_ldu_set_coeffs(unap_hip_ldu A, const double* diag, const double* up
     });
 }
 
-int unap_hip_ldu_spmv(unap_hip_ldu A, const double* psi, double* Apsi, int where) {
+int unap_hip_ldu_set_topology(unap_hip_ldu A, int64_t nf, const int64_t* la, const int64_t* ua) {
     return guard([&] {
         checkLdu(A);
+        HostLdu& H = A->o.host;
+        const int64_t n = H.n;
+        need(nf >= 0 && 2 * nf < (1LL << 31), "sizes out of range");
+        need(nf == 0 || (la && ua), "null array");
+        for (int64_t f = 0; f < nf; ++f)
+            need(la[f] >= 0 && la[f] < n && ua[f] >= 0 && ua[f] < n && la[f] != ua[f], "face address out of range");
+        if (Ctx* c = Ctx::peek()) c->sync();  // device copies about to be freed may be in use
+        LduObj& o = A->o;
+        const bool sameCount = nf == H.nFaces();
+        H.l.assign(la, la + nf);
+        H.u.assign(ua, ua + nf);
+        if (!sameCount) {
+            // the coefficient arrays no longer match the faces (the
+            // reference keeps them until setMatrixCoeffients): refuse use
+            H.up.clear();
+            H.lo.clear();
+            o.coeffsPending = true;
+        }
+        for (LduView& v : o.views) {
+            v.d64.reset();
+            v.d32.reset();
+            v.layout.reset();
+        }
+        o.agglCache.reset();
+        o.topoGen += 1;
+        o.coeffGen += 1;
+        levelCounts(H, o.nFwdLevels, o.nBwdLevels);
+    });
+}
+
+int unap_hip_ldu_spmv(unap_hip_ldu A, const double* psi, double* Apsi, int where) {
+    return guard([&] {
+        checkLduReady(A);
         need(psi && Apsi, "null vector");
         DevMatrix<double>& M = A->o.dev64(ORDER_STREAM);
         DBuf<double> xi(M.n), yi(M.n);
@@ -1003,7 +1044,7 @@ int unap_hip_ldu_spmv(unap_hip_ldu A, const double* psi, double* Apsi, int where
 
 int unap_hip_ldu_spmv_noalloc(unap_hip_ldu A, const double* psi, double* Apsi) {
     return guard([&] {
-        checkLdu(A);
+        checkLduReady(A);
         need(psi && Apsi, "null vector");
         const int n = A->o.n();
         need(n <= kNoAllocMaxCells, "unap_hip_ldu_spmv_noalloc: the matrix has more cells than the scratch holds "
@@ -1068,13 +1109,14 @@ static AmgParams toParams(const unap_hip_amg_params* p) {
 int unap_hip_precond_create(unap_hip_ldu A, int type, const unap_hip_amg_params* amg, const double* fw,
                             unap_hip_precond* out) {
     return guard([&] {
-        checkLdu(A);
+        checkLduReady(A);
         need(out != nullptr, "out is null");
         need(type >= 0 && type <= 4, "unknown preconditioner type");
         std::unique_ptr<unap_hip_precond_s> P(new unap_hip_precond_s);
         PrecondObj& o = P->o;
         o.type = type;
         o.A = &A->o;
+        o.topoGen = A->o.topoGen;
         const HostLdu& H = A->o.host;
         Ctx& c = Ctx::get();
         if (type == UNAP_HIP_PRECOND_DIC || type == UNAP_HIP_PRECOND_DILU) o.order = ORDER_SWEEP;
@@ -1133,6 +1175,8 @@ int unap_hip_precond_apply(unap_hip_precond P, const double* r, double* w, int w
     return guard([&] {
         need(P && r && w, "null argument");
         PrecondObj& o = P->o;
+        if (o.A->coeffsPending || o.topoGen != o.A->topoGen)
+            throw Error(UNAP_HIP_ERR_STATE, "preconditioner was built for an earlier topology of this matrix");
         DevMatrix<double>& M = o.A->dev64(o.order);
         DBuf<double> ri(M.n), wi(M.n);
         toInternal(M, r, where, ri.get(), stageBuf());
@@ -1219,6 +1263,7 @@ int unap_hip_smoother_smooth(unap_hip_smoother S, unap_hip_ldu A, double* x, con
     return guard([&] {
         need(S && A && x && b, "null argument");
         need(nSweeps >= 0, "negative sweep count");
+        checkLduReady(A);
         DevMatrix<double>& M = A->o.dev64(S->o.type == UNAP_HIP_SMOOTHER_GS ? ORDER_SWEEP : ORDER_STREAM);
         DBuf<double> xi(M.n), bi(M.n);
         toInternal(M, x, where, xi.get(), stageBuf());
@@ -1242,6 +1287,7 @@ int unap_hip_smoother_smooth(unap_hip_smoother S, unap_hip_ldu A, double* x, con
 int unap_hip_eigen_estimate(unap_hip_ldu A, double* x, const double* b, int64_t nPCGs, int where, double* lam) {
     return guard([&] {
         need(A && x && b && lam, "null argument");
+        checkLduReady(A);
         DevMatrix<double>& M = A->o.dev64(ORDER_STREAM);
         DBuf<double> xi(M.n), bi(M.n), p(M.n), w(M.n), r(M.n);
         toInternal(M, x, where, xi.get(), stageBuf());
@@ -1259,6 +1305,7 @@ int unap_hip_aggl_create(unap_hip_ldu A, const double* weights, int64_t maxLevel
     return guard([&] {
         need(A && weights && out, "null argument");
         need(maxLevels >= 1 && maxLevels <= 60 && nCoarsest >= 1, "bad level limits");
+        checkLduReady(A);
         auto* g = new unap_hip_aggl_s;
         g->A = &A->o;
         try {
@@ -1508,6 +1555,7 @@ extern "C" {
 int unap_hip_profile_kernel(unap_hip_ldu A, int kernel, int precision, int64_t reps, double* avgMs, double* bytes) {
     return guard([&] {
         need(A && avgMs && bytes, "null argument");
+        checkLduReady(A);
         need(reps >= 1, "reps must be positive");
         need(precision == 64 || precision == 32, "precision must be 64 or 32");
         if (precision == 64)
@@ -1572,9 +1620,12 @@ int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const do
                           unap_hip_perf* perf) {
     return guard([&] {
         need(S && A && x && b, "null argument");
+        checkLduReady(A);
         SolverObj& o = S->o;
         LduObj& L = A->o;
         need(!o.P || o.P->A == &L, "preconditioner was built for another matrix");
+        if (o.P && o.P->topoGen != L.topoGen)
+            throw Error(UNAP_HIP_ERR_STATE, "preconditioner was built for an earlier topology of this matrix");
         need(!(o.P && o.P->amg32 && o.type != UNAP_HIP_SOLVER_FCG), "the FP32 AMG preconditioner needs FCG");
         need(!(o.P && o.P->lu32 && o.type != UNAP_HIP_SOLVER_FCG && o.type != UNAP_HIP_SOLVER_PBICGSTAB &&
                o.type != UNAP_HIP_SOLVER_FBICGSTAB),
@@ -1595,11 +1646,12 @@ int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const do
         }
         if (o.type == UNAP_HIP_SOLVER_MG && o.mgHasWeights)
             need((int64_t)o.mgWeights.size() == L.host.nFaces(), "MG weights were given for a matrix with another face count");
-        if (o.type == UNAP_HIP_SOLVER_MG && (!o.mgH || o.mgFor != L.id)) {
+        if (o.type == UNAP_HIP_SOLVER_MG && (!o.mgH || o.mgFor != L.id || o.mgTopoGen != L.topoGen)) {
             auto ag = agglFor(L, o.mgHasWeights ? o.mgWeights.data() : nullptr, true, o.mg.maxLevels, o.mg.nCoarsest);
             o.mgH.reset(new Hierarchy<double>);
             o.mgH->build(L, ag, o.mg);
             o.mgFor = L.id;
+            o.mgTopoGen = L.topoGen;
         }
         // coefficients updated since the hierarchy last ran: new Galerkin levels
         if (o.type == UNAP_HIP_SOLVER_MG) o.mgH->refresh();

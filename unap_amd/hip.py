@@ -55,6 +55,7 @@ SIGNATURES = {
     "unap_hip_ldu_destroy": (ctypes.c_int, [_vp]),
     "unap_hip_ldu_info": (ctypes.c_int, [_vp, _i64p, _i64p, _intp, _i64p, _i64p]),
     "unap_hip_ldu_set_coeffs": (ctypes.c_int, [_vp, _f64p, _f64p, _f64p]),
+    "unap_hip_ldu_set_topology": (ctypes.c_int, [_vp, _i64, _i64p, _i64p]),
     "unap_hip_ldu_spmv": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "unap_hip_ldu_spmv_noalloc": (ctypes.c_int, [_vp, _vp, _vp]),
     "unap_hip_amg_params_default": (ctypes.c_int, [ctypes.POINTER(AmgParams)]),
@@ -280,6 +281,12 @@ class LduMatrix:
         self._up = np.ascontiguousarray(upper, np.float64)
         self._lo = None if lower is None or lower is upper else np.ascontiguousarray(lower, np.float64)
         _check(lib().unap_hip_ldu_set_coeffs(self.h, _p(self._d), _p(self._up), _p(self._lo)))
+
+    def set_topology(self, lower_addr, upper_addr):
+        """setMatrixTopology: new addressing (same cells); coefficients untouched."""
+        self._l = np.ascontiguousarray(lower_addr, np.int64)
+        self._u = np.ascontiguousarray(upper_addr, np.int64)
+        _check(lib().unap_hip_ldu_set_topology(self.h, self._u.shape[0], _p(self._l, _i64p), _p(self._u, _i64p)))
 
     def spmv(self, x, out=None):
         xp, where, keep = _vec(x)
