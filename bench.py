@@ -50,7 +50,8 @@ def main():
     ap.add_argument("--mixed-n-coarsest", type=int, default=2000,
                     help="coarsest-level threshold of the mixed algorithm (dense FP64 inverse)")
     ap.add_argument("--pair-steps", type=int, default=2, help="pairwise steps per level of the mixed hierarchy")
-    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=3,
+                    help="timed solves of each FP64 leg (median; one untimed warm-up solve before them)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-size", type=int, default=None, help="edge length of the CPU sample (default: --size)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_256.json"),
@@ -151,6 +152,7 @@ def main():
         print(json.dumps({"mixed_s": times32, "iterations": p32["nIterations"], "hierarchy": levels32}), flush=True)
         return
 
+    solve(S64)  # warm-up
     times64 = []
     for k in range(max(1, args.fp64_steps)):
         dt, p64 = solve(S64)
@@ -158,6 +160,7 @@ def main():
         log(f"fp64 step {k}: {dt * 1e3:.1f} ms, {p64['nIterations']} iterations, converged={p64['converged']}")
     x64 = x.numpy()
     t64 = float(np.median(times64))
+    solve(S64f)  # warm-up
     times64f = []
     for k in range(max(1, args.fp64_steps)):
         dt, p64f = solve(S64f)
@@ -230,9 +233,11 @@ def main():
         "hierarchy_levels": levels32,
         "mixed": {"s": round(t32, 5), "device_ms": round(ms_dev32, 3), "iterations": p32["nIterations"],
                   "converged": p32["converged"], "err_vs_fp64_1e-12": err32, "true_rel_residual": true_res32},
-        "fp64_drop_in": {"s": round(t64, 5), "iterations": p64["nIterations"], "converged": p64["converged"],
+        "fp64_drop_in": {"s": round(t64, 5), "steps_s": [round(t, 5) for t in times64],
+                         "iterations": p64["nIterations"], "converged": p64["converged"],
                          "err_vs_fp64_1e-12": err64},
-        "fp64_same_algorithm": {"s": round(t64f, 5), "iterations": p64f["nIterations"],
+        "fp64_same_algorithm": {"s": round(t64f, 5), "steps_s": [round(t, 5) for t in times64f],
+                                "iterations": p64f["nIterations"],
                                 "converged": p64f["converged"], "err_vs_fp64_1e-12": err64f},
         "speedup_mixed_vs_fp64": round(t64 / t32, 3),
         "speedup_mixed_vs_fp64_same_algorithm": round(t64f / t32, 3),
@@ -398,13 +403,27 @@ def batch_bench(nsys, launches, with_cpu):
     its = np.array([p["nIterations"] for p in perfs])
     conv = all(p["converged"] for p in perfs)
     p50 = float(np.percentile(lat, 50))
+    # roofline of the launch: algorithmic bytes per system and iteration
+    # (each array a step reads or writes, once, unpadded), summed over the
+    # systems' iterations, over the median launch time.  PCG + FP32 DIC:
+    # 152 n + 40 f (ELL Amul 28 n + 24 f, FP32 sweep tables 36 n + 16 f, dots
+    # and updates 88 n); PBiCGStab + FP32 DILU: 296 n + 80 f (two of each).
+    alg = 0.0
+    for sp, sysm, it in zip(specs, systems, its):
+        n, f = sysm.n, sysm.nfaces
+        per = (152 * n + 40 * f) if sp[3] == meshgen.POISSON_LOGUNIF else (296 * n + 80 * f)
+        alg += per * int(it)
+    gbs = alg / (p50 * 1e-3) / 1e9
     log(f"batch launch ms p50 {p50:.2f} p90 {np.percentile(lat, 90):.2f} p99 {np.percentile(lat, 99):.2f}; "
-        f"{nsys / (p50 * 1e-3):.0f} systems/s; all converged {conv}; max true rel. residual {max(res):.2e}")
+        f"{nsys / (p50 * 1e-3):.0f} systems/s; all converged {conv}; max true rel. residual {max(res):.2e}; "
+        f"{gbs:.0f} GB/s algorithmic ({gbs / PEAK_HBM_GBS:.1%} of 8 TB/s)")
     out = {"workload": f"C5: {nsys} permuted LDU boxes, edges 8..48 ({ncells} cells), 60% PCG+DIC / "
                        "40% PBiCGStab+DILU, FP32 preconditioners, FP64 Krylov, rel.tol 1e-8, one launch",
            "systems_per_s": round(nsys / (p50 * 1e-3), 1), "launches": launches,
            "latency_ms": {"p50": round(p50, 3), "p90": round(float(np.percentile(lat, 90)), 3),
                           "p99": round(float(np.percentile(lat, 99)), 3), "min": round(float(lat.min()), 3)},
+           "roofline": {"bound": "hbm", "alg_bytes_per_launch": alg, "achieved": round(gbs, 1),
+                        "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4)},
            "all_converged": conv, "max_true_rel_residual": max(res),
            "iterations": {"min": int(its.min()), "median": float(np.median(its)), "max": int(its.max())},
            "autotuned": {"block": bs, "workgroups_per_cu": wg, "tune_s": round(t_tune, 2)},

@@ -43,8 +43,8 @@ def alg_bytes(kernel, lv, iters, launches=None):
         return iters * sum((4 * s + 4) * n + (2 * s + 8) * f for n, f in [lv[0]] + lv[1:L - 1])
     if "BJacobiScale<float>" in k:
         return iters * sum(5 * s * n for n, _ in lv[1:L - 1])
-    if "BPsiScaled<float>" in k:
-        return iters * 7 * s * n0
+    if "BPsiScaled<float>" in k:  # fixed cycle: psi is written, not read (reads fc, fres, Apsi, d; writes fc, psi)
+        return iters * 6 * s * n0
     if "BRestrict<float, float>" in k:
         return iters * sum((s + 4) * (lv[i][0] + lv[i + 1][0]) for i in range(L))
     if "BProlong<float>" in k:
@@ -56,8 +56,8 @@ def alg_bytes(kernel, lv, iters, launches=None):
     if "kDenseSolve" in k:
         nc = lv[-1][0]
         return iters * (8 * nc * nc + 8 * nc)
-    if "BFcgRz<float>" in k:
-        return iters * 16 * n0
+    if "BFcgRz<float>" in k:      # r (FP64), z (FP32), q = A p (FP64)
+        return iters * 20 * n0
     if "BFcgP<float>" in k:
         return iters * 20 * n0
     if "BAmulDot" in k:
