@@ -389,8 +389,8 @@ def batch_bench(nsys, launches, with_cpu):
     t0 = time.time()
     _, perfs = B.solve(b, x=x, rel_tol=1e-8, max_iter=1000)  # first solve autotunes
     t_tune = time.time() - t0
-    bs, wg = B.config()
-    log(f"batch autotune {t_tune:.1f}s -> block {bs}, {wg} workgroups/CU")
+    bs, wg, rw = B.config()
+    log(f"batch autotune {t_tune:.1f}s -> block {bs}, {wg} workgroups/CU, {rw} rows a wavefront pass")
     lat = []
     for _ in range(launches):
         x.zero()
@@ -426,7 +426,7 @@ def batch_bench(nsys, launches, with_cpu):
                         "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4)},
            "all_converged": conv, "max_true_rel_residual": max(res),
            "iterations": {"min": int(its.min()), "median": float(np.median(its)), "max": int(its.max())},
-           "autotuned": {"block": bs, "workgroups_per_cu": wg, "tune_s": round(t_tune, 2)},
+           "autotuned": {"block": bs, "workgroups_per_cu": wg, "rows_per_wavefront": rw, "tune_s": round(t_tune, 2)},
            "setup_s": {"generate": round(t_gen, 2), "create": round(t_create, 2)}}
     if with_cpu:
         out["cpu_baseline"] = batch_cpu_baseline(systems, solvers, p50)

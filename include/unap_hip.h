@@ -53,6 +53,12 @@ const char* unap_hip_last_error(void);
 #define UNAP_HIP_REDUCE_REFERENCE 1
 int unap_hip_set_reduction_order(int order);
 
+/* Solver iterations from the second on replayed from a HIP graph of one
+   iteration (1, default) or launched one kernel at a time (0).  No reference
+   counterpart; the results are identical either way.  The environment
+   variable UNAP_HIP_GRAPHS=0 also turns the replay off. */
+int unap_hip_set_graph_replay(int on);
+
 /* Device name and compute-unit count of the current device. */
 int unap_hip_device_info(char* name, int64_t name_len, int64_t* compute_units);
 
@@ -292,12 +298,16 @@ int unap_hip_batch_destroy(unap_hip_batch B);
 int unap_hip_batch_solve(unap_hip_batch B, double* x, const double* b, int where, int64_t max_iter,
                          double rel_tol, double tol, unap_hip_perf* perf);
 /* Device time of the last batch launch (HIP events), and the launch shape the
-   autotuner chose on the first solve (threads per workgroup, workgroups per CU). */
+   autotuner chose on the first solve (threads per workgroup, workgroups per
+   CU, rows a wavefront covers per pass of the vector loops). */
 int unap_hip_batch_last_ms(unap_hip_batch B, double* ms);
-int unap_hip_batch_config(unap_hip_batch B, int64_t* block_size, int64_t* wg_per_cu);
+int unap_hip_batch_config(unap_hip_batch B, int64_t* block_size, int64_t* wg_per_cu,
+                          int64_t* rows_per_wavefront);
 /* Set the launch shape and skip the autotuner: block_size 256, 512 or 1024,
-   wg_per_cu >= 1 with block_size * wg_per_cu <= 2048. */
-int unap_hip_batch_set_config(unap_hip_batch B, int64_t block_size, int64_t wg_per_cu);
+   wg_per_cu >= 1 with block_size * wg_per_cu <= 2048, rows_per_wavefront 64,
+   128 or 256 (results do not depend on it). */
+int unap_hip_batch_set_config(unap_hip_batch B, int64_t block_size, int64_t wg_per_cu,
+                              int64_t rows_per_wavefront);
 
 #ifdef __cplusplus
 }

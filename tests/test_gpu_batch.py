@@ -47,27 +47,32 @@ def test_batch_against_oracle(oracle, nsys, lo, hi, seed):
         e_batch = np.linalg.norm(xi - x_ref) / np.linalg.norm(x_ref)
         e_fp64 = np.linalg.norm(xo - x_ref) / np.linalg.norm(x_ref)
         assert e_batch <= max(100 * e_fp64, 1e-6), (i, e_batch, e_fp64)
-    bs, wg = B.config()
-    assert bs in (256, 512, 1024) and wg in (1, 2, 4)
+    bs, wg, rw = B.config()
+    assert bs in (256, 512, 1024) and wg in (1, 2, 4) and rw in (64, 128, 256)
 
 
 def test_batch_set_config():
     systems, solvers = _batch(12, 4, 8, 3)
     B = hip.Batch(systems, solvers)
-    for args in [(128, 1), (256, 0), (1024, 4)]:
+    for args in [(128, 1, 256), (256, 0, 256), (1024, 4, 256), (256, 1, 96)]:
         with pytest.raises(hip.UnapHipError) as e:
             B.set_config(*args)
         assert e.value.code == -1
     b = np.concatenate([s.b for s in systems])
-    B.set_config(256, 2)
+    B.set_config(256, 2, 256)
     x1, p1 = B.solve(b, rel_tol=1e-8)
-    assert B.config() == (256, 2)  # no autotuning after set_config
-    B.set_config(1024, 1)
+    assert B.config() == (256, 2, 256)  # no autotuning after set_config
+    B.set_config(1024, 1, 256)
     x2, p2 = B.solve(b, rel_tol=1e-8)
     # one workgroup a system either way: the block size changes the order of
     # the block sums only
     assert all(p["converged"] for p in p1 + p2)
     assert np.allclose(x1, x2, rtol=0, atol=1e-6 * np.abs(x1).max())
+    # rows per wavefront pass: the same per-thread order, the same bits
+    for rw in (64, 128):
+        B.set_config(1024, 1, rw)
+        x3, p3 = B.solve(b, rel_tol=1e-8)
+        assert np.array_equal(x3, x2) and [p["nIterations"] for p in p3] == [p["nIterations"] for p in p2]
 
 
 def test_batch_device_pointers_and_repeat():

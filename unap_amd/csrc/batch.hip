@@ -7,8 +7,8 @@
 // the system's level schedule with workgroup barriers between levels.
 // Workgroups take systems from an atomic ticket in a largest-first order
 // (longest-processing-time scheduling), and exit when the queue is empty.
-// The block size and workgroups per CU are picked by a small autotuner that
-// times whole batch solves.
+// The block size, workgroups per CU and rows per wavefront pass are picked by
+// a small autotuner that times whole batch solves.
 #include "unap_hip.h"
 
 #include <hip/hip_runtime.h>
@@ -219,20 +219,20 @@ __device__ __forceinline__ void batchPrecond(const SysView& s, const double* r, 
     batchSweep<false>(s.bwd, s.bs, s.nBwd, r, w);
 }
 
-// Vector loops over rows i = t, t + nt, ... (< n), kVecRows rows at a time:
-// the rows' loads first (rows past n re-read row n - 1 and are not used),
-// then their work in row order, the same order as one row at a time, so the
-// sums are unchanged.  Stores between rows otherwise keep each row's loads
-// from starting before the previous row is done.
-constexpr int kVecRows = 4;
-template <class V, class Load, class Use>
+// Vector loops over rows i = t, t + nt, ... (< n), VR rows at a time (a
+// wavefront covers 64 VR rows a pass; VR is one of the autotuned launch
+// parameters): the rows' loads first (rows past n re-read row n - 1 and are
+// not used), then their work in row order, the same order as one row at a
+// time, so the sums do not depend on VR.  Stores between rows otherwise keep
+// each row's loads from starting before the previous row is done.
+template <int VR, class V, class Load, class Use>
 __device__ __forceinline__ void vecRows(int t, int n, int nt, Load&& load, Use&& use) {
-    for (int i0 = t; i0 < n; i0 += kVecRows * nt) {
-        V v[kVecRows];
+    for (int i0 = t; i0 < n; i0 += VR * nt) {
+        V v[VR];
 #pragma unroll
-        for (int u = 0; u < kVecRows; ++u) v[u] = load(min(i0 + u * nt, n - 1));
+        for (int u = 0; u < VR; ++u) v[u] = load(min(i0 + u * nt, n - 1));
 #pragma unroll
-        for (int u = 0; u < kVecRows; ++u)
+        for (int u = 0; u < VR; ++u)
             if (i0 + u * nt < n) use(i0 + u * nt, v[u]);
     }
 }
@@ -259,6 +259,7 @@ __device__ __forceinline__ bool bconv(double fin, double init, double tol, doubl
     return fin < tol || (relTol > kSmallD && fin <= relTol * init);
 }
 
+template <int VR>
 __device__ void batchPCG(const BatchArgs& a, const SysView& s, int off, unap_hip_perf* pf, double* red) {
     const int n = s.n, t = threadIdx.x, nt = blockDim.x;
     double* x = a.x + off;
@@ -285,7 +286,7 @@ __device__ void batchPCG(const BatchArgs& a, const SysView& s, int off, unap_hip
     for (;;) {
         batchPrecond(s, r, w32);
         double rz = 0.0, zq = 0.0;
-        vecRows<V3>(t, n, nt, [&](int i) { return V3{(double)w32[i], r[i], w[i]}; },
+        vecRows<VR, V3>(t, n, nt, [&](int i) { return V3{(double)w32[i], r[i], w[i]}; },
                     [&](int, const V3& v) {
                         rz += v.a * v.b;
                         zq += v.a * v.c;
@@ -297,7 +298,7 @@ __device__ void batchPCG(const BatchArgs& a, const SysView& s, int off, unap_hip
             break;
         }
         const double beta = -zq / pq;
-        vecRows<V2>(t, n, nt, [&](int i) { return V2{(double)w32[i], p[i]}; },
+        vecRows<VR, V2>(t, n, nt, [&](int i) { return V2{(double)w32[i], p[i]}; },
                     [&](int i, const V2& v) { p[i] = (it == 0) ? v.a : v.a + beta * v.b; });
         __syncthreads();
         acc = 0.0;
@@ -309,7 +310,7 @@ __device__ void batchPCG(const BatchArgs& a, const SysView& s, int off, unap_hip
         pq = bsum(acc, red);
         const double alpha = rho / pq;
         acc = 0.0;
-        vecRows<V4>(t, n, nt, [&](int i) { return V4{x[i], p[i], r[i], w[i]}; },
+        vecRows<VR, V4>(t, n, nt, [&](int i) { return V4{x[i], p[i], r[i], w[i]}; },
                     [&](int i, const V4& v) {
                         x[i] = v.a + alpha * v.b;
                         const double ri = v.c - alpha * v.d;
@@ -335,6 +336,7 @@ __device__ void batchPCG(const BatchArgs& a, const SysView& s, int off, unap_hip
     }
 }
 
+template <int VR>
 __device__ void batchBiCG(const BatchArgs& a, const SysView& s, int off, unap_hip_perf* pf, double* red) {
     const int n = s.n, t = threadIdx.x, nt = blockDim.x;
     const long long T = a.total;
@@ -367,7 +369,7 @@ __device__ void batchBiCG(const BatchArgs& a, const SysView& s, int off, unap_hi
     for (;;) {
         const double rhoOld = rho;
         acc = 0.0;
-        vecRows<V2>(t, n, nt, [&](int i) { return V2{r0[i], r[i]}; }, [&](int, const V2& v) { acc += v.a * v.b; });
+        vecRows<VR, V2>(t, n, nt, [&](int i) { return V2{r0[i], r[i]}; }, [&](int, const V2& v) { acc += v.a * v.b; });
         rho = bsum(acc, red);
         // The FP32 preconditioner is not an exactly linear operator, and the
         // shadow residual loses bi-orthogonality with it: iteration counts
@@ -389,11 +391,11 @@ __device__ void batchBiCG(const BatchArgs& a, const SysView& s, int off, unap_hi
             break;
         }
         const double beta = (rho / rhoOld) * (alpha / omega);
-        vecRows<V3>(t, n, nt, [&](int i) { return V3{r[i], p[i], Ay[i]}; },
+        vecRows<VR, V3>(t, n, nt, [&](int i) { return V3{r[i], p[i], Ay[i]}; },
                     [&](int i, const V3& v) { p[i] = (k == 0) ? v.a : v.a + beta * (v.b - omega * v.c); });
         __syncthreads();
         batchPrecond(s, p, w32);
-        vecRows<V1>(t, n, nt, [&](int i) { return V1{(double)w32[i]}; }, [&](int i, const V1& v) { y[i] = v.a; });
+        vecRows<VR, V1>(t, n, nt, [&](int i) { return V1{(double)w32[i]}; }, [&](int i, const V1& v) { y[i] = v.a; });
         __syncthreads();
         acc = 0.0;
         for (int i = t; i < n; i += nt) {
@@ -403,7 +405,7 @@ __device__ void batchBiCG(const BatchArgs& a, const SysView& s, int off, unap_hi
         }
         alpha = rho / bsum(acc, red);
         acc = 0.0;
-        vecRows<V2>(t, n, nt, [&](int i) { return V2{r[i], Ay[i]}; },
+        vecRows<VR, V2>(t, n, nt, [&](int i) { return V2{r[i], Ay[i]}; },
                     [&](int i, const V2& v) {
                         const double si = v.a - alpha * v.b;
                         sv[i] = si;
@@ -417,7 +419,7 @@ __device__ void batchBiCG(const BatchArgs& a, const SysView& s, int off, unap_hi
             break;
         }
         batchPrecond(s, sv, w32);
-        vecRows<V1>(t, n, nt, [&](int i) { return V1{(double)w32[i]}; }, [&](int i, const V1& v) { z[i] = v.a; });
+        vecRows<VR, V1>(t, n, nt, [&](int i) { return V1{(double)w32[i]}; }, [&](int i, const V1& v) { z[i] = v.a; });
         __syncthreads();
         double tt = 0.0, ts = 0.0;
         for (int i = t; i < n; i += nt) {
@@ -429,7 +431,7 @@ __device__ void batchBiCG(const BatchArgs& a, const SysView& s, int off, unap_hi
         bsum2(tt, ts, red);
         omega = ts / tt;
         acc = 0.0;
-        vecRows<V5>(t, n, nt, [&](int i) { return V5{y[i], z[i], sv[i], t_[i], x[i]}; },
+        vecRows<VR, V5>(t, n, nt, [&](int i) { return V5{y[i], z[i], sv[i], t_[i], x[i]}; },
                     [&](int i, const V5& v) {
                         x[i] = v.e + (alpha * v.a + omega * v.b);
                         const double ri = v.c - omega * v.d;
@@ -456,7 +458,7 @@ __device__ void batchBiCG(const BatchArgs& a, const SysView& s, int off, unap_hi
     }
 }
 
-template <int BS>
+template <int BS, int VR>
 __global__ void __launch_bounds__(BS) kBatchSolve(BatchArgs a) {
     __shared__ double red[32];
     __shared__ int ticket;
@@ -479,9 +481,9 @@ __global__ void __launch_bounds__(BS) kBatchSolve(BatchArgs a) {
         s.nFwd = d.nFwd;
         s.nBwd = d.nBwd;
         if (d.type == UNAP_HIP_SOLVER_PCG)
-            batchPCG(a, s, d.cellOff, a.perf + a.order[q], red);
+            batchPCG<VR>(a, s, d.cellOff, a.perf + a.order[q], red);
         else
-            batchBiCG(a, s, d.cellOff, a.perf + a.order[q], red);
+            batchBiCG<VR>(a, s, d.cellOff, a.perf + a.order[q], red);
         __syncthreads();
     }
 }
@@ -511,6 +513,7 @@ struct unap_hip_batch_s {
     DBuf<int> tabCol[2];
     DBuf<unap_hip_perf> perf;
     int blockSize = 0, wgPerCU = 0;  // 0 = not tuned yet
+    int rowsPerThread = 4;           // vector-loop rows a thread (64 x this rows a wavefront pass)
     double lastMs = 0;
     int cus = 0;
 };
@@ -532,7 +535,16 @@ int bguard(F&& f) {
     }
 }
 
-void launchBatch(unap_hip_batch_s& B, double relTol, double tol, long long maxIter, int bs, int wgPerCU) {
+template <int BS>
+void launchShape(const BatchArgs& a, int grid, int vr, hipStream_t st) {
+    switch (vr) {
+        case 1: kBatchSolve<BS, 1><<<grid, BS, 0, st>>>(a); break;
+        case 2: kBatchSolve<BS, 2><<<grid, BS, 0, st>>>(a); break;
+        default: kBatchSolve<BS, 4><<<grid, BS, 0, st>>>(a); break;
+    }
+}
+
+void launchBatch(unap_hip_batch_s& B, double relTol, double tol, long long maxIter, int bs, int wgPerCU, int vr) {
     Ctx& c = Ctx::get();
     BatchArgs a;
     a.sys = B.sys.get();
@@ -564,9 +576,9 @@ void launchBatch(unap_hip_batch_s& B, double relTol, double tol, long long maxIt
     kResetTicket<<<1, 1, 0, c.stream>>>(B.ticket.get());
     UH_LAUNCHED();
     switch (bs) {
-        case 256: kBatchSolve<256><<<grid, 256, 0, c.stream>>>(a); break;
-        case 512: kBatchSolve<512><<<grid, 512, 0, c.stream>>>(a); break;
-        default: kBatchSolve<1024><<<grid, 1024, 0, c.stream>>>(a); break;
+        case 256: launchShape<256>(a, grid, vr, c.stream); break;
+        case 512: launchShape<512>(a, grid, vr, c.stream); break;
+        default: launchShape<1024>(a, grid, vr, c.stream); break;
     }
     UH_LAUNCHED();
 }
@@ -791,10 +803,10 @@ int unap_hip_batch_solve(unap_hip_batch B, double* x, const double* b, int where
         hipEvent_t e0, e1;
         UH_CHECK(hipEventCreate(&e0));
         UH_CHECK(hipEventCreate(&e1));
-        auto timed = [&](int bsz, int wg) {
+        auto timed = [&](int bsz, int wg, int vr) {
             UH_CHECK(hipMemcpyAsync(B->x.get(), x0.get(), sizeof(double) * T, hipMemcpyDeviceToDevice, c.stream));
             UH_CHECK(hipEventRecord(e0, c.stream));
-            launchBatch(*B, relTol, tol, maxIter, bsz, wg);
+            launchBatch(*B, relTol, tol, maxIter, bsz, wg, vr);
             UH_CHECK(hipEventRecord(e1, c.stream));
             UH_CHECK(hipEventSynchronize(e1));
             float ms = 0;
@@ -802,20 +814,30 @@ int unap_hip_batch_solve(unap_hip_batch B, double* x, const double* b, int where
             return (double)ms;
         };
         if (B->blockSize == 0) {
-            // autotune: block shape x workgroups per CU, one full batch solve each
+            // autotune, one full batch solve a candidate: block shape x
+            // workgroups per CU at 4 rows a thread, then the rows a thread
+            // (1, 2, 4: 64, 128, 256 rows a wavefront pass) at the best shape
             double best = 1e300;
+            B->rowsPerThread = 4;
             for (int bsz : {256, 512, 1024})
                 for (int wg : {1, 2, 4}) {
                     if (bsz * wg > 2048) continue;
-                    const double ms = timed(bsz, wg);
+                    const double ms = timed(bsz, wg, 4);
                     if (ms < best) {
                         best = ms;
                         B->blockSize = bsz;
                         B->wgPerCU = wg;
                     }
                 }
+            for (int vr : {1, 2}) {
+                const double ms = timed(B->blockSize, B->wgPerCU, vr);
+                if (ms < best) {
+                    best = ms;
+                    B->rowsPerThread = vr;
+                }
+            }
         }
-        B->lastMs = timed(B->blockSize, B->wgPerCU);
+        B->lastMs = timed(B->blockSize, B->wgPerCU, B->rowsPerThread);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (where == UNAP_HIP_DEVICE) {
@@ -838,23 +860,27 @@ int unap_hip_batch_last_ms(unap_hip_batch B, double* ms) {
     });
 }
 
-int unap_hip_batch_config(unap_hip_batch B, int64_t* block_size, int64_t* wg_per_cu) {
+int unap_hip_batch_config(unap_hip_batch B, int64_t* block_size, int64_t* wg_per_cu, int64_t* rows_per_wavefront) {
     return bguard([&] {
         if (!B) throw Error(UNAP_HIP_ERR_ARG, "null argument");
         if (block_size) *block_size = B->blockSize;
         if (wg_per_cu) *wg_per_cu = B->wgPerCU;
+        if (rows_per_wavefront) *rows_per_wavefront = 64 * B->rowsPerThread;
     });
 }
 
-int unap_hip_batch_set_config(unap_hip_batch B, int64_t block_size, int64_t wg_per_cu) {
+int unap_hip_batch_set_config(unap_hip_batch B, int64_t block_size, int64_t wg_per_cu, int64_t rows_per_wavefront) {
     return bguard([&] {
         if (!B) throw Error(UNAP_HIP_ERR_ARG, "null argument");
         if (!(block_size == 256 || block_size == 512 || block_size == 1024))
             throw Error(UNAP_HIP_ERR_ARG, "block_size must be 256, 512 or 1024");
         if (wg_per_cu < 1 || block_size * wg_per_cu > 2048)
             throw Error(UNAP_HIP_ERR_ARG, "wg_per_cu must be >= 1 with block_size * wg_per_cu <= 2048");
+        if (!(rows_per_wavefront == 64 || rows_per_wavefront == 128 || rows_per_wavefront == 256))
+            throw Error(UNAP_HIP_ERR_ARG, "rows_per_wavefront must be 64, 128 or 256");
         B->blockSize = (int)block_size;
         B->wgPerCU = (int)wg_per_cu;
+        B->rowsPerThread = (int)(rows_per_wavefront / 64);
     });
 }
 

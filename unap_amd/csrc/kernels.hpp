@@ -49,7 +49,8 @@ struct KState {
     double tol, relTol;
     long long maxIter, minIter, nIter;
     int converged, singular, done, stop;
-    long long exitIt;
+    long long exitIt;  // the iteration that left on ||s|| (-1: none)
+    int exitApplied;   // its x += alpha y has been applied
     // flexible BiCGStab: shadow-residual restart state
     double r0norm;
     long long kShadow;       // iterations since the shadow residual was (re)set
@@ -338,6 +339,7 @@ static __global__ void kStateInit(KState* st, double tol, double relTol, long lo
     st->nIter = 0;
     st->converged = st->singular = st->done = st->stop = 0;
     st->exitIt = -1;
+    st->exitApplied = 0;
     st->r0norm = 0.0;
     st->kShadow = 0;
     st->restart = 0;
@@ -407,11 +409,10 @@ struct FinBicgAlpha {  // PBiCGStab.cpp:254-269
 };
 struct FinBicgS {  // PBiCGStab.cpp:276-348: converged on ||s|| -> early exit
     KState* st;
-    long long it;
     __device__ void operator()(double (&t)[1]) const {
         st->finalRes = sqrt(t[0]) + kSmallD;
         if (checkConv(st, st->nIter + 1)) {
-            st->exitIt = it;
+            st->exitIt = st->nIter;
             st->nIter += 1;
             if (st->hist && st->nIter < st->histCap) st->hist[st->nIter] = st->finalRes;
             st->done = 1;

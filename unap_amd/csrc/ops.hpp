@@ -232,13 +232,17 @@ struct BBicgXR {  // PBiCGStab.cpp:388-396
     }
     __device__ void terms(int i, double (&q)[1]) const { q[0] = r[i] * r[i]; }
 };
-// early exit x += alpha yA (PBiCGStab.cpp:298-312), only in the iteration that
-// took it
-static __global__ void kBicgEarlyX(double* x, const double* y, const KState* st, long long it, int n) {
-    if (st->exitIt != it) return;
+// early exit x += alpha yA (PBiCGStab.cpp:298-312), once: every later launch
+// (iterations enqueued past the exit) finds it applied.  The state needs no
+// iteration index from the host, so the step replays inside a graph.
+static __global__ void kBicgEarlyX(double* x, const double* y, const KState* st, int n) {
+    if (st->exitIt < 0 || st->exitApplied) return;
     const double al = st->alpha;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         x[i] += al * y[i];
+}
+static __global__ void kBicgExitApplied(KState* st) {
+    if (threadIdx.x == 0 && st->exitIt >= 0) st->exitApplied = 1;
 }
 
 // ================================================================ FCG bodies

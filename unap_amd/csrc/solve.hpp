@@ -198,8 +198,7 @@ void chebySweeps(DevMatrix<T>& M, T* x, const T* b, int nSweeps, const KState* e
             launchRows(BCheby<T>{M.view(), cur, nxt, b, w, est, s}, M.n, sk);
         std::swap(cur, nxt);
     }
-    if (cur != x)
-        UH_CHECK(hipMemcpyAsync(x, cur, sizeof(T) * M.n, hipMemcpyDeviceToDevice, Ctx::get().stream));
+    if (cur != x) launchRows(BCopy<T, T>{cur, x}, M.n, sk);  // a kernel, not a copy node: graph-capturable
 }
 
 // Acf = A f with the sums of scalingFactor, in the matrix's rowAmul shape.
@@ -394,6 +393,21 @@ struct PrecondObj {
 void applyPrecond64(PrecondObj& P, const double* r, double* w, Skip sk);
 
 // ----------------------------------------------------------------- solvers
+// An instantiated graph of one solver iteration and what it was captured for.
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned long long> key;
+    GraphCache() = default;
+    GraphCache(const GraphCache&) = delete;
+    GraphCache& operator=(const GraphCache&) = delete;
+    ~GraphCache() { reset(); }
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+    }
+};
+
 struct SolverObj {
     int type = 1;
     PrecondObj* P = nullptr;
@@ -408,6 +422,8 @@ struct SolverObj {
     std::unique_ptr<Hierarchy<double>> mgH;
     unsigned long long mgFor = 0;  // id of the matrix mgH was built for
     long long mgTopoGen = -1;      // and its topology generation
+    // graph of one iteration, replayed by later iterations and solves
+    GraphCache graph;
     // results
     std::vector<double> history;
     double lastMs = 0;

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -17,6 +18,7 @@
 namespace unap_hip {
 
 static thread_local std::string g_lastError;
+static int g_graphMode = 1;  // unap_hip_set_graph_replay
 void setLastError(const std::string& m) { g_lastError = m; }
 
 static Ctx* g_ctx = nullptr;
@@ -625,6 +627,22 @@ static int nextBatch(const KState& h, int cur) {
     return std::max(1, std::min(b, 16));
 }
 
+// Iterations replayed from a HIP graph.  Iteration 0 of every solve runs as
+// ordinary launches: it does the one-off work whose host side differs (the
+// reference cycle's first-call eigenvalue estimates, the backward-sweep
+// copies built on first use).  Iteration 1 is captured and the graph replays
+// every later one: the kernels read their iteration-dependent scalars from
+// the device state, so one graph serves them all.  The graph is kept on the
+// solver and reused by later solves while everything its kernel arguments
+// point at is unchanged (the key below).
+static bool graphsEnabled() {
+    static const int on = [] {
+        const char* e = std::getenv("UNAP_HIP_GRAPHS");
+        return (e && e[0] == '0') ? 0 : 1;
+    }();
+    return on && g_graphMode;
+}
+
 struct Driver {
     SolverObj& S;
     LduObj& A;
@@ -632,15 +650,42 @@ struct Driver {
     SolveBufs& B;
     KState* st;
     int n;
+    std::vector<unsigned long long> key;  // what the captured launches depend on
     Driver(SolverObj& s, LduObj& a, DevMatrix<double>& m, SolveBufs& b)
         : S(s), A(a), M(m), B(b), st(b.st.get()), n(a.n()) {}
 
     template <class Enqueue>
     void iterate(Enqueue&& oneIteration) {
+        Ctx& c = Ctx::get();
+        GraphCache& G = S.graph;
+        const bool graphs = graphsEnabled() && !key.empty();
+        if (!graphs || G.key != key) G.reset();
         long long it = 0;
         int batch = 1;
         for (;;) {
-            for (int k = 0; k < batch; ++k) oneIteration(it++);
+            for (int k = 0; k < batch; ++k, ++it) {
+                if (!graphs || it == 0) {
+                    oneIteration(it);
+                    continue;
+                }
+                if (!G.exec) {
+                    hipGraph_t g = nullptr;
+                    UH_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+                    try {
+                        oneIteration(it);
+                    } catch (...) {
+                        (void)hipStreamEndCapture(c.stream, &g);
+                        if (g) (void)hipGraphDestroy(g);
+                        throw;
+                    }
+                    UH_CHECK(hipStreamEndCapture(c.stream, &g));
+                    const hipError_t e = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    UH_CHECK(e);
+                    G.key = key;
+                }
+                UH_CHECK(hipGraphLaunch(G.exec, c.stream));
+            }
             const KState& h = readState(st);
             if (h.done) return;
             batch = nextBatch(h, batch);
@@ -701,10 +746,12 @@ struct Driver {
             }
             precondInto(p, y, sk);
             launchReduce<1>(BAmulDotR0{M.view(), y, r0, Ay}, n, M.iperm.get(), FinBicgAlpha{st}, sk);
-            launchReduce<1>(BBicgS{r, Ay, s, st}, n, M.iperm.get(), FinBicgS{st, it}, sk);
+            launchReduce<1>(BBicgS{r, Ay, s, st}, n, M.iperm.get(), FinBicgS{st}, sk);
             {
                 const Launch L = rowLaunch(n);
-                kBicgEarlyX<<<L.grid, kBlock, 0, Ctx::get().stream>>>(x, y, st, it, n);
+                kBicgEarlyX<<<L.grid, kBlock, 0, Ctx::get().stream>>>(x, y, st, n);
+                UH_LAUNCHED();
+                kBicgExitApplied<<<1, 64, 0, Ctx::get().stream>>>(st);
                 UH_LAUNCHED();
             }
             precondInto(s, z, sk);
@@ -913,6 +960,13 @@ void checkLduReady(unap_hip_ldu A) {
 extern "C" {
 
 const char* unap_hip_last_error(void) { return g_lastError.c_str(); }
+
+int unap_hip_set_graph_replay(int on) {
+    return guard([&] {
+        need(on == 0 || on == 1, "graph replay is 0 or 1");
+        g_graphMode = on;
+    });
+}
 
 int unap_hip_set_reduction_order(int order) {
     return guard([&] {
@@ -1668,6 +1722,10 @@ int unap_hip_solver_solve(unap_hip_solver S, unap_hip_ldu A, double* x, const do
         kStateInit<<<1, 1, 0, c.stream>>>(st, o.tol, o.relTol, o.maxIter, o.minIter, S->bufs.hist.get(), histCap);
         UH_LAUNCHED();
         Driver d(o, L, M, S->bufs);
+        d.key = {L.id, (unsigned long long)L.coeffGen, (unsigned long long)L.topoGen,
+                 (unsigned long long)(uintptr_t)o.P, (unsigned long long)o.type,
+                 (unsigned long long)c.serialReductions, (unsigned long long)(uintptr_t)S->bufs.x.get(),
+                 (unsigned long long)(uintptr_t)S->bufs.hist.get(), (unsigned long long)(uintptr_t)o.mgH.get()};
         switch (o.type) {
             case UNAP_HIP_SOLVER_PCG: d.pcg(); break;
             case UNAP_HIP_SOLVER_PBICGSTAB: d.bicg(); break;

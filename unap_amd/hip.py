@@ -51,6 +51,7 @@ SIGNATURES = {
     "unap_hip_last_error": (ctypes.c_char_p, []),
     "unap_hip_device_info": (ctypes.c_int, [ctypes.c_char_p, _i64, _i64p]),
     "unap_hip_set_reduction_order": (ctypes.c_int, [ctypes.c_int]),
+    "unap_hip_set_graph_replay": (ctypes.c_int, [ctypes.c_int]),
     "unap_hip_ldu_create": (ctypes.c_int, [_i64, _i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(_vp)]),
     "unap_hip_ldu_destroy": (ctypes.c_int, [_vp]),
     "unap_hip_ldu_info": (ctypes.c_int, [_vp, _i64p, _i64p, _intp, _i64p, _i64p]),
@@ -94,8 +95,8 @@ SIGNATURES = {
     "unap_hip_batch_solve": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _i64, ctypes.c_double, ctypes.c_double,
                                             ctypes.POINTER(Perf)]),
     "unap_hip_batch_last_ms": (ctypes.c_int, [_vp, _f64p]),
-    "unap_hip_batch_config": (ctypes.c_int, [_vp, _i64p, _i64p]),
-    "unap_hip_batch_set_config": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64]),
+    "unap_hip_batch_config": (ctypes.c_int, [_vp, _i64p, _i64p, _i64p]),
+    "unap_hip_batch_set_config": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
 }
 
 _lib = None
@@ -145,6 +146,11 @@ REDUCE_TREE, REDUCE_REFERENCE = 0, 1
 def set_reduction_order(order):
     """REDUCE_TREE (fast, default) or REDUCE_REFERENCE (the reference's serial sums)."""
     _check(lib().unap_hip_set_reduction_order(order))
+
+
+def set_graph_replay(on):
+    """Replay solver iterations from a HIP graph (True, default) or launch them kernel by kernel."""
+    _check(lib().unap_hip_set_graph_replay(1 if on else 0))
 
 
 def device_info():
@@ -547,10 +553,11 @@ class Batch:
         return v.value
 
     def config(self):
-        bs, wg = _i64(), _i64()
-        _check(lib().unap_hip_batch_config(self.h, ctypes.byref(bs), ctypes.byref(wg)))
-        return bs.value, wg.value
+        """(threads per workgroup, workgroups per CU, rows per wavefront pass)."""
+        bs, wg, rw = _i64(), _i64(), _i64()
+        _check(lib().unap_hip_batch_config(self.h, ctypes.byref(bs), ctypes.byref(wg), ctypes.byref(rw)))
+        return bs.value, wg.value, rw.value
 
-    def set_config(self, block_size, wg_per_cu):
+    def set_config(self, block_size, wg_per_cu, rows_per_wavefront=256):
         """Fix the launch shape (the autotuner then does not run)."""
-        _check(lib().unap_hip_batch_set_config(self.h, block_size, wg_per_cu))
+        _check(lib().unap_hip_batch_set_config(self.h, block_size, wg_per_cu, rows_per_wavefront))
