@@ -78,6 +78,11 @@ int unap_hip_ldu_destroy(unap_hip_ldu A);
    of the forward and backward sweeps (0 when faces are not owner-sorted). */
 int unap_hip_ldu_info(unap_hip_ldu A, int64_t* n_cells, int64_t* n_faces, int* symmetric,
                       int64_t* fwd_levels, int64_t* bwd_levels);
+/* lowerAddr() / upperAddr() / lower() / diag() / upper() (lduMatrix.hpp:122-129):
+   host copies of the matrix's arrays (any pointer may be NULL; lower is upper
+   for a symmetric matrix). */
+int unap_hip_ldu_get(unap_hip_ldu A, int64_t* lower_addr, int64_t* upper_addr, double* lower, double* diag,
+                     double* upper);
 /* setMatrixCoeffients(diag, upper, lower) (lduMatrix.cpp:508-529): new values,
    same addressing; lower == upper or NULL keeps one coefficient array. */
 int unap_hip_ldu_set_coeffs(unap_hip_ldu A, const double* diag, const double* upper,
@@ -279,6 +284,43 @@ int unap_hip_solver_last_ms(unap_hip_solver S, double* ms);
 #define UNAP_HIP_KERNEL_COPY 8       /* y = x, a plain copy (achievable-bandwidth yardstick) */
 int unap_hip_profile_kernel(unap_hip_ldu A, int kernel, int precision, int64_t reps, double* avg_ms,
                             double* bytes);
+
+/* ------------------------------------------------------- formats and files
+ * Host-side construction of matrices from other formats and text files
+ * (one domain; no processor interfaces). */
+/* coo2ldu after reorderCOO (src/tools/matrixConversion/coo2ldu.cpp:3-115,
+   matrixConversion.cpp:38-99): diagonal entries to diag, entries above it to
+   faces in row-major order (owner = row < neighbour = column), entries below
+   it to lower of the matching face (not read when symmetric).  Any entry
+   order (reorderCOO needs the entries grouped by row; a stable row sort comes
+   first here); indices from `base` (0 or 1); duplicate entries and a structure that
+   is not symmetric are refused (UNAP_HIP_ERR_ARG).  Output arrays may be NULL
+   (call once with NULL to learn n_faces, then with arrays of that length;
+   diag of length n). */
+int unap_hip_coo_to_ldu(int64_t n, int64_t nnz, const int64_t* rows, const int64_t* cols, const double* vals,
+                        int symmetric, int64_t base, int64_t* n_faces, int64_t* lower_addr, int64_t* upper_addr,
+                        double* lower, double* diag, double* upper);
+/* A matrix from COO, or from CSR (csr2ldu, coo2ldu.cpp:117-148: row_start of
+   length n + 1), by the conversion above. */
+int unap_hip_ldu_create_coo(int64_t n, int64_t nnz, const int64_t* rows, const int64_t* cols, const double* vals,
+                            int symmetric, int64_t base, unap_hip_ldu* out);
+int unap_hip_ldu_create_csr(int64_t n, const int64_t* row_start, const int64_t* cols, const double* vals,
+                            int symmetric, int64_t base, unap_hip_ldu* out);
+/* The OpenFOAM-style text dump: constructLDUMatrixFromOpenFOAM /
+   constructVectorFromOpenFOAM (src/tools/readData/readFromOpenFOAM/
+   readFromOpenFOAM.cpp:9-192) and printLDUMatrix / printVector
+   (src/tools/printData/printUNAP/printUNAP.cpp:8-59, printUNAP.hpp:28-40),
+   the same text, at `path` as given (the reference appends "_<rank>.txt"). */
+int unap_hip_ldu_read_openfoam(const char* path, unap_hip_ldu* out);
+int unap_hip_ldu_write_openfoam(unap_hip_ldu A, const char* path);
+int unap_hip_vector_read_openfoam(const char* path, int64_t n, double* out);
+int unap_hip_vector_write_openfoam(const char* path, int64_t n, const double* v);
+/* The Hypre IJ text format: constructLDUMatrixFromHypre (asymmetric storage)
+   and constructVectorFromHypre (src/tools/readData/readFromHypre/
+   readFromHypre.cpp:11-254, 454-523), one domain (columns outside the row
+   range are refused). */
+int unap_hip_ldu_read_hypre(const char* path, unap_hip_ldu* out);
+int unap_hip_vector_read_hypre(const char* path, int64_t n, double* out);
 
 /* ------------------------------------------------------------------ batches
  * Many independent LDU systems solved in one persistent kernel launch: one

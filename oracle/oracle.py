@@ -70,7 +70,7 @@ class CpuSolvePath:
               _f64p, _f64p, _f64p, _f64p, ctypes.c_int64, _i64p])
 
         _sig(L, P + "ldu_set_coeffs", None, [_vp, _f64p, _f64p, _f64p])
-        _sig(L, P + "solver_create", _vp, [_vp, ctypes.c_int, ctypes.c_int, _i64p])
+        _sig(L, P + "solver_create", _vp, [_vp, ctypes.c_int, ctypes.c_int, _i64p, _f64p])
         _sig(L, P + "solver_run", ctypes.c_int,
              [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _f64p, _f64p, _f64p,
               _f64p, ctypes.c_int64, _i64p])
@@ -81,6 +81,44 @@ class CpuSolvePath:
 
     def fn(self, name):
         return getattr(self.lib, self.prefix + name)
+
+    # ---- formats (the compiled reference's tools only) -----------------
+    def _tools(self):
+        L, P = self.lib, self.prefix
+        if getattr(self, "_tools_bound", False):
+            return
+        _sig(L, P + "ldu_get", None, [_vp, _i64p, _i64p, _i64p, _i64p, _i64p, _f64p, _f64p, _f64p])
+        _sig(L, P + "coo2ldu", _vp, [ctypes.c_int64, ctypes.c_int64, _i64p, _i64p, _f64p, ctypes.c_int])
+        _sig(L, P + "csr2ldu", _vp, [ctypes.c_int64, _i64p, _i64p, _f64p, ctypes.c_int])
+        _sig(L, P + "reorder_coo", None, [_f64p, _i64p, _i64p, ctypes.c_int64, ctypes.c_int64])
+        _sig(L, P + "reorder_uface", None, [_i64p, _i64p, ctypes.c_int64, ctypes.c_int64, _i64p])
+        _sig(L, P + "reorder_lface", None, [_i64p, _i64p, ctypes.c_int64, ctypes.c_int64, _i64p])
+        _sig(L, P + "reorder_value", None, [_f64p, _i64p, ctypes.c_int64])
+        _sig(L, P + "write_openfoam", None, [_vp, ctypes.c_char_p])
+        _sig(L, P + "write_vector_openfoam", None, [_f64p, ctypes.c_int64, ctypes.c_char_p])
+        _sig(L, P + "read_openfoam", _vp, [ctypes.c_char_p])
+        _sig(L, P + "read_vector_openfoam", None, [ctypes.c_char_p, ctypes.c_int64, _f64p])
+        _sig(L, P + "read_hypre", _vp, [ctypes.c_char_p])
+        _sig(L, P + "read_vector_hypre", None, [ctypes.c_char_p, ctypes.c_int64, _f64p])
+        self._tools_bound = True
+
+    def ldu_arrays(self, h):
+        """(lowerAddr, upperAddr, lower, diag, upper, symmetric) of a reference lduMatrix handle."""
+        self._tools()
+        n, nf, sy = (np.zeros(1, np.int64) for _ in range(3))
+        self.fn("ldu_get")(h, _p(n, _i64p), _p(nf, _i64p), _p(sy, _i64p), None, None, None, None, None)
+        m = int(nf[0])
+        l = np.empty(m, np.int64)
+        u = np.empty(m, np.int64)
+        lo = np.empty(m)
+        d = np.empty(int(n[0]))
+        up = np.empty(m)
+        self.fn("ldu_get")(h, None, None, None, _p(l, _i64p), _p(u, _i64p), _p(lo), _p(d), _p(up))
+        return l, u, lo, d, up, bool(sy[0])
+
+    def tool(self, name):
+        self._tools()
+        return self.fn(name)
 
     # ---- matrix --------------------------------------------------------
     def matrix(self, sys):
@@ -188,20 +226,22 @@ class CpuMatrix:
         self._vals = (d, up, lo)  # the oracle restatement copies; the reference shim copies too
         self.path.fn("ldu_set_coeffs")(self.h, _p(d), _p(up), _p(lo))
 
-    def persistent(self, solver, precond=None, mg=None):
-        """A solver whose preconditioner / hierarchy is built now and kept."""
-        return CpuPersistentSolver(self, solver, precond, mg)
+    def persistent(self, solver, precond=None, mg=None, weights=None):
+        """A solver whose preconditioner / hierarchy is built now and kept
+        (MG: agglomerated with the given face weights, default |upper| / ||upper||)."""
+        return CpuPersistentSolver(self, solver, precond, mg, weights)
 
     def solve(self, solver, precond, b, x0=None, max_iter=1000, min_iter=0, rel_tol=1e-6, tol=0.0,
-              mg=None, hist_cap=2048):
+              mg=None, hist_cap=2048, weights=None):
         x = np.zeros(self.n) if x0 is None else np.array(x0, np.float64)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float64)
         b = np.ascontiguousarray(b, np.float64)
         perf = np.zeros(6)
         hist = np.zeros(hist_cap)
         hl = np.zeros(1, np.int64)
         mga = None if mg is None else np.asarray(mg, np.int64)
         if solver == "MG":
-            rc = self.path.fn("mg_solve")(self.h, _p(mga, _i64p), None, max_iter, min_iter, rel_tol, tol,
+            rc = self.path.fn("mg_solve")(self.h, _p(mga, _i64p), _p(w), max_iter, min_iter, rel_tol, tol,
                                           _p(x), _p(b), _p(perf), _p(hist), hist_cap, _p(hl, _i64p))
         else:
             rc = self.path.fn("krylov_solve")(self.h, SOLVER[solver], PRECOND[precond], _p(mga, _i64p),
@@ -233,12 +273,13 @@ class CpuPrecond:
 
 
 class CpuPersistentSolver:
-    def __init__(self, M, solver, precond, mg):
+    def __init__(self, M, solver, precond, mg, weights=None):
         self.M = M
         self.mga = None if mg is None else np.asarray(mg, np.int64)
+        self.w = None if weights is None else np.ascontiguousarray(weights, np.float64)
         kind = 3 if solver == "MG" else SOLVER[solver]
         pre = 0 if precond is None else PRECOND[precond]
-        self.h = M.path.fn("solver_create")(M.h, kind, pre, _p(self.mga, _i64p))
+        self.h = M.path.fn("solver_create")(M.h, kind, pre, _p(self.mga, _i64p), _p(self.w))
         if not self.h:
             raise RuntimeError("solver_create failed")
 

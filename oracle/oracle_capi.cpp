@@ -243,15 +243,19 @@ struct SolverBox {
     std::vector<std::unique_ptr<Smoother>> sm;
 };
 
-void* uo_solver_create(void* Av, int solver, int precond, const int64_t* mg) {
+void* uo_solver_create(void* Av, int solver, int precond, const int64_t* mg, const double* weights) {
     const Ldu& A = *(Ldu*)Av;
     std::unique_ptr<SolverBox> b(new SolverBox);
     b->type = solver;
     if (solver == 3) {
         if (!mg) return nullptr;
         std::vector<double> w(A.nFaces());
-        const double avg = std::sqrt(sumSqr(A.upper(), A.nFaces()));
-        for (int64_t f = 0; f < A.nFaces(); ++f) w[f] = std::fabs(A.upper()[f]) / avg;
+        if (weights) {
+            for (int64_t f = 0; f < A.nFaces(); ++f) w[f] = weights[f];
+        } else {
+            const double avg = std::sqrt(sumSqr(A.upper(), A.nFaces()));
+            for (int64_t f = 0; f < A.nFaces(); ++f) w[f] = std::fabs(A.upper()[f]) / avg;
+        }
         b->G.reset(new Agglomeration(A));
         b->G->nCoarsest = mg[1];
         b->G->agglomerate(w.data());

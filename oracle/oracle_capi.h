@@ -86,10 +86,11 @@ void OFN(ldu_set_coeffs)(void* A, const double* diag, const double* upper, const
    between shows what the reference's objects keep (DIC/DILU/diagonal rD,
    agglomeration maps) and what they recompute (Galerkin matrices on every
    MGSolver::solve).  solver 1, 2, 4 with precond as krylov_solve, or 3 (MG,
-   weights |upper| / ||upper||_2); mg[] as there.  Each run re-arms the
+   agglomerated with the given face weights, or |upper| / ||upper||_2 when
+   weights is null); mg[] as there.  Each run re-arms the
    Chebyshev smoothers (initSmoothers, MG.cpp:54-59) and returns the history
    as krylov_solve does. */
-void* OFN(solver_create)(void* A, int solver, int precond, const int64_t* mg);
+void* OFN(solver_create)(void* A, int solver, int precond, const int64_t* mg, const double* weights);
 int OFN(solver_run)(void* S, void* A, int64_t maxIter, int64_t minIter, double relTol, double tol,
                     double* x, const double* b, double* perf, double* history, int64_t histCap,
                     int64_t* histLen);

@@ -23,6 +23,10 @@
 #include "lduDiagPrecond.hpp"
 #include "lduGaussSeidelSmoother.hpp"
 #include "lduMatrix.hpp"
+#include "matrixConversion.hpp"
+#include "printUNAP.hpp"
+#include "readFromHypre.hpp"
+#include "readFromOpenFOAM.hpp"
 #include "unapMPI.hpp"
 
 using namespace UNAP;
@@ -365,15 +369,19 @@ struct RefSolverBox {
     matrix::solver& solver() const { return mg ? (matrix::solver&)*mg : *S; }
 };
 
-void* ur_solver_create(void* Av, int solver, int precond, const int64_t* mg) {
+void* ur_solver_create(void* Av, int solver, int precond, const int64_t* mg, const double* weights) {
     ensureMPI();
     const lduMatrix& A = *(lduMatrix*)Av;
     RefSolverBox* b = new RefSolverBox;
     b->type = solver;
     if (solver == 3) {
         scalarField w(A.upper().size());
-        const scalar avg = A.upper().SumSqrt();
-        for (label i = 0; i < w.size(); ++i) w[i] = std::fabs(A.upper()[i]) / avg;
+        if (weights) {
+            for (label i = 0; i < w.size(); ++i) w[i] = weights[i];
+        } else {
+            const scalar avg = A.upper().SumSqrt();
+            for (label i = 0; i < w.size(); ++i) w[i] = std::fabs(A.upper()[i]) / avg;
+        }
         b->G.reset(new lduAgglomeration(A));
         b->G->SET_nCellsInCoarsestLevel(mg[1]);
         b->G->agglomerate(w);
@@ -485,6 +493,80 @@ int ur_mg_solve(void* Av, const int64_t* mg, const double* weights, int64_t maxI
     };
     return solveWithHistory(make, A, maxIter, minIter, relTol, tol, x, b, perf, history, histCap,
                             histLen);
+}
+
+// ---------------------------------------- formats and files (reference tools)
+// Arrays of an lduMatrix (lower is upper when symmetric).
+void ur_ldu_get(void* Av, int64_t* n, int64_t* nf, int64_t* symm, int64_t* l, int64_t* u, double* lower,
+                double* diag, double* upper) {
+    const lduMatrix& A = *(lduMatrix*)Av;
+    const label m = A.upperAddr().size();
+    if (n) *n = A.size();
+    if (nf) *nf = m;
+    if (symm) *symm = A.symm() ? 1 : 0;
+    for (label f = 0; f < m; ++f) {
+        if (l) l[f] = A.lowerAddr()[f];
+        if (u) u[f] = A.upperAddr()[f];
+        if (upper) upper[f] = A.upper()[f];
+        if (lower) lower[f] = A.lower()[f];
+    }
+    if (diag)
+        for (label i = 0; i < A.size(); ++i) diag[i] = A.diag()[i];
+}
+
+// reorderCOO then coo2ldu (the sequence of coo2ldumatrixcreat_,
+// fortranInterface.cpp:74-120) on 0-based copies.
+void* ur_coo2ldu(int64_t n, int64_t nnz, const int64_t* rows, const int64_t* cols, const double* vals, int symm) {
+    ensureMPI();
+    std::vector<label> r(rows, rows + nnz), c(cols, cols + nnz);
+    std::vector<scalar> v(vals, vals + nnz);
+    reorderCOO(v.data(), r.data(), c.data(), n, nnz);
+    return &coo2ldu(v.data(), r.data(), c.data(), n, nnz, symm != 0);
+}
+void* ur_csr2ldu(int64_t n, const int64_t* rowStart, const int64_t* cols, const double* vals, int symm) {
+    ensureMPI();
+    std::vector<label> rs(rowStart, rowStart + n + 1), c(cols, cols + rowStart[n]);
+    return &csr2ldu(vals, rs.data(), c.data(), n, rowStart[n], symm != 0);
+}
+void ur_reorder_coo(double* val, int64_t* row, int64_t* col, int64_t n, int64_t size) {
+    reorderCOO(val, row, col, n, size);
+}
+void ur_reorder_uface(int64_t* row, int64_t* col, int64_t n, int64_t size, int64_t* newOrder) {
+    reorderUFace(row, col, n, size, newOrder);
+}
+void ur_reorder_lface(int64_t* row, int64_t* col, int64_t n, int64_t size, int64_t* newOrder) {
+    reorderLFace(row, col, n, size, newOrder);
+}
+void ur_reorder_value(double* val, const int64_t* newOrder, int64_t size) { reorderValue(val, newOrder, size); }
+
+// printLDUMatrix / printVector write "<name>_<rank>.txt".
+void ur_write_openfoam(void* Av, const char* name) { printLDUMatrix(*(lduMatrix*)Av, name); }
+void ur_write_vector_openfoam(const double* v, int64_t n, const char* name) {
+    scalarField f(v, n);
+    printVector(f, name);
+}
+void* ur_read_openfoam(const char* path) {
+    lduMatrix* A = new lduMatrix;
+    constructLDUMatrixFromOpenFOAM(*A, path);
+    return A;
+}
+void ur_read_vector_openfoam(const char* path, int64_t n, double* out) {
+    scalarField b(n);
+    constructVectorFromOpenFOAM(b, path);
+    for (label i = 0; i < n; ++i) out[i] = b[i];
+}
+// constructLDUMatrixFromHypre also writes "A_diag_<rank>.txt" in the working
+// directory (readFromHypre.cpp:226).
+void* ur_read_hypre(const char* path) {
+    ensureMPI();
+    lduMatrix* A = new lduMatrix;
+    constructLDUMatrixFromHypre(*A, path);
+    return A;
+}
+void ur_read_vector_hypre(const char* path, int64_t n, double* out) {
+    scalarField b(n);
+    constructVectorFromHypre(b, path);
+    for (label i = 0; i < n; ++i) out[i] = b[i];
 }
 
 }  // extern "C"

@@ -1025,6 +1025,22 @@ int unap_hip_ldu_info(unap_hip_ldu A, int64_t* n, int64_t* nf, int* symm, int64_
     });
 }
 
+int unap_hip_ldu_get(unap_hip_ldu A, int64_t* la, int64_t* ua, double* lower, double* diag, double* upper) {
+    return guard([&] {
+        checkLduReady(A);
+        const HostLdu& H = A->o.host;
+        const int nf = H.nFaces();
+        const double* lw = H.lower();
+        for (int f = 0; f < nf; ++f) {
+            if (la) la[f] = H.l[f];
+            if (ua) ua[f] = H.u[f];
+            if (upper) upper[f] = H.up[f];
+            if (lower) lower[f] = lw[f];
+        }
+        if (diag) std::memcpy(diag, H.d.data(), sizeof(double) * H.n);
+    });
+}
+
 int unap_hip_ldu_set_coeffs(unap_hip_ldu A, const double* diag, const double* upper, const double* lower) {
     return guard([&] {
         checkLdu(A);

@@ -57,6 +57,18 @@ SIGNATURES = {
     "unap_hip_ldu_info": (ctypes.c_int, [_vp, _i64p, _i64p, _intp, _i64p, _i64p]),
     "unap_hip_ldu_set_coeffs": (ctypes.c_int, [_vp, _f64p, _f64p, _f64p]),
     "unap_hip_ldu_set_topology": (ctypes.c_int, [_vp, _i64, _i64p, _i64p]),
+    "unap_hip_ldu_get": (ctypes.c_int, [_vp, _i64p, _i64p, _f64p, _f64p, _f64p]),
+    "unap_hip_coo_to_ldu": (ctypes.c_int, [_i64, _i64, _i64p, _i64p, _f64p, ctypes.c_int, _i64, _i64p, _i64p, _i64p,
+                                           _f64p, _f64p, _f64p]),
+    "unap_hip_ldu_create_coo": (ctypes.c_int, [_i64, _i64, _i64p, _i64p, _f64p, ctypes.c_int, _i64,
+                                               ctypes.POINTER(_vp)]),
+    "unap_hip_ldu_create_csr": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, ctypes.c_int, _i64, ctypes.POINTER(_vp)]),
+    "unap_hip_ldu_read_openfoam": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "unap_hip_ldu_write_openfoam": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "unap_hip_vector_read_openfoam": (ctypes.c_int, [ctypes.c_char_p, _i64, _f64p]),
+    "unap_hip_vector_write_openfoam": (ctypes.c_int, [ctypes.c_char_p, _i64, _f64p]),
+    "unap_hip_ldu_read_hypre": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "unap_hip_vector_read_hypre": (ctypes.c_int, [ctypes.c_char_p, _i64, _f64p]),
     "unap_hip_ldu_spmv": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "unap_hip_ldu_spmv_noalloc": (ctypes.c_int, [_vp, _vp, _vp]),
     "unap_hip_amg_params_default": (ctypes.c_int, [ctypes.POINTER(AmgParams)]),
@@ -230,6 +242,43 @@ def synchronize():
         raise RuntimeError("hipDeviceSynchronize failed")
 
 
+def coo_to_ldu(n, rows, cols, vals, symmetric=False, base=0):
+    """(lowerAddr, upperAddr, lower, diag, upper) of the COO matrix, as coo2ldu gives them."""
+    r = np.ascontiguousarray(rows, np.int64)
+    c = np.ascontiguousarray(cols, np.int64)
+    v = np.ascontiguousarray(vals, np.float64)
+    nf = _i64()
+    L = lib()
+    _check(L.unap_hip_coo_to_ldu(n, r.size, _p(r, _i64p), _p(c, _i64p), _p(v), int(symmetric), base,
+                                 ctypes.byref(nf), None, None, None, None, None))
+    m = nf.value
+    l = np.empty(m, np.int64)
+    u = np.empty(m, np.int64)
+    lo = np.empty(m)
+    d = np.empty(n)
+    up = np.empty(m)
+    _check(L.unap_hip_coo_to_ldu(n, r.size, _p(r, _i64p), _p(c, _i64p), _p(v), int(symmetric), base,
+                                 ctypes.byref(nf), _p(l, _i64p), _p(u, _i64p), _p(lo), _p(d), _p(up)))
+    return l, u, lo, d, up
+
+
+def read_vector_openfoam(path, n):
+    out = np.empty(n)
+    _check(lib().unap_hip_vector_read_openfoam(os.fsencode(path), n, _p(out)))
+    return out
+
+
+def write_vector_openfoam(path, v):
+    v = np.ascontiguousarray(v, np.float64)
+    _check(lib().unap_hip_vector_write_openfoam(os.fsencode(path), v.size, _p(v)))
+
+
+def read_vector_hypre(path, n):
+    out = np.empty(n)
+    _check(lib().unap_hip_vector_read_hypre(os.fsencode(path), n, _p(out)))
+    return out
+
+
 def dense_inverse_spd(a):
     """Inverse of a dense SPD matrix by the GPU's blocked Gauss-Jordan."""
     a = np.ascontiguousarray(a, np.float64)
@@ -267,6 +316,67 @@ class LduMatrix:
     @classmethod
     def from_system(cls, s):
         return cls(s.n, s.l, s.u, None if s.symmetric else s.lower, s.diag, s.upper)
+
+    @classmethod
+    def _from_handle(cls, h):
+        A = cls.__new__(cls)
+        A.h = h
+        n, nf, sy = _i64(), _i64(), ctypes.c_int()
+        _check(lib().unap_hip_ldu_info(h, ctypes.byref(n), ctypes.byref(nf), ctypes.byref(sy), None, None))
+        A.n = n.value
+        A._l, A._u, A._lo, A._d, A._up = A._arrays(nf.value, bool(sy.value))
+        return A
+
+    def _arrays(self, nf, symm):
+        l = np.empty(nf, np.int64)
+        u = np.empty(nf, np.int64)
+        lo = np.empty(nf)
+        d = np.empty(self.n)
+        up = np.empty(nf)
+        _check(lib().unap_hip_ldu_get(self.h, _p(l, _i64p), _p(u, _i64p), _p(lo), _p(d), _p(up)))
+        return l, u, (None if symm else lo), d, up
+
+    def arrays(self):
+        """(lowerAddr, upperAddr, lower, diag, upper) host copies; lower is None when symmetric."""
+        info = self.info()
+        return self._arrays(info["nfaces"], info["symmetric"])
+
+    @classmethod
+    def from_coo(cls, n, rows, cols, vals, symmetric=False, base=0):
+        """coo2ldu after reorderCOO (coo2ldu.cpp, matrixConversion.cpp)."""
+        r = np.ascontiguousarray(rows, np.int64)
+        c = np.ascontiguousarray(cols, np.int64)
+        v = np.ascontiguousarray(vals, np.float64)
+        h = _vp()
+        _check(lib().unap_hip_ldu_create_coo(n, r.size, _p(r, _i64p), _p(c, _i64p), _p(v), int(symmetric), base,
+                                             ctypes.byref(h)))
+        return cls._from_handle(h)
+
+    @classmethod
+    def from_csr(cls, n, row_start, cols, vals, symmetric=False, base=0):
+        """csr2ldu (coo2ldu.cpp:117-148)."""
+        rs = np.ascontiguousarray(row_start, np.int64)
+        c = np.ascontiguousarray(cols, np.int64)
+        v = np.ascontiguousarray(vals, np.float64)
+        h = _vp()
+        _check(lib().unap_hip_ldu_create_csr(n, _p(rs, _i64p), _p(c, _i64p), _p(v), int(symmetric), base,
+                                             ctypes.byref(h)))
+        return cls._from_handle(h)
+
+    @classmethod
+    def read_openfoam(cls, path):
+        h = _vp()
+        _check(lib().unap_hip_ldu_read_openfoam(os.fsencode(path), ctypes.byref(h)))
+        return cls._from_handle(h)
+
+    @classmethod
+    def read_hypre(cls, path):
+        h = _vp()
+        _check(lib().unap_hip_ldu_read_hypre(os.fsencode(path), ctypes.byref(h)))
+        return cls._from_handle(h)
+
+    def write_openfoam(self, path):
+        _check(lib().unap_hip_ldu_write_openfoam(self.h, os.fsencode(path)))
 
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib is not None:
