@@ -104,7 +104,7 @@ def main():
     t0 = time.time()
     A = hip.LduMatrix.from_system(s)
     t_create = time.time() - t0
-    info = A.info()
+    info = A.info(levels=False)
     log(f"ldu_create {t_create:.1f}s", info)
 
     common = dict(smoother=2, n_vcycles=1, n_post_sweeps=4, n_finest_sweeps=4)

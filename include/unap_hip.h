@@ -75,7 +75,9 @@ int unap_hip_ldu_create(int64_t n_cells, int64_t n_faces, const int64_t* lower_a
                         const double* upper, unap_hip_ldu* out);
 int unap_hip_ldu_destroy(unap_hip_ldu A);
 /* size() / nFaces / symm() (lduMatrix.hpp:210-231) and the level-schedule depth
-   of the forward and backward sweeps (0 when faces are not owner-sorted). */
+   of the forward and backward sweeps (0 when faces are not owner-sorted).  The
+   depths are computed on the first call that asks for them (a pass over every
+   row); pass NULL for them to skip it. */
 int unap_hip_ldu_info(unap_hip_ldu A, int64_t* n_cells, int64_t* n_faces, int* symmetric,
                       int64_t* fwd_levels, int64_t* bwd_levels);
 /* lowerAddr() / upperAddr() / lower() / diag() / upper() (lduMatrix.hpp:122-129):
@@ -321,6 +323,26 @@ int unap_hip_vector_write_openfoam(const char* path, int64_t n, const double* v)
    range are refused). */
 int unap_hip_ldu_read_hypre(const char* path, unap_hip_ldu* out);
 int unap_hip_vector_read_hypre(const char* path, int64_t n, double* out);
+/* Reverse Cuthill-McKee order of the cell graph (ReorderingSSM and
+   rcmLDU_nowrite, src/tools/rcm/cooRCM.cpp:79-235, 477-637): new_of_old[c] is
+   cell c's new label.  Each connected component starts at its unvisited cell
+   of least degree (lowest id among equals); neighbours join by ascending
+   degree, equal degrees in face order; the sequence is then reversed.  (The
+   reference's neighbour sort uses a comparator that is not an ordering, so its
+   order is defined only where that does not matter.) */
+int unap_hip_rcm_order(int64_t n, int64_t n_faces, const int64_t* lower_addr, const int64_t* upper_addr,
+                       int64_t* new_of_old);
+/* rcmLDU_rewrite (cooRCM.cpp:639-785) for a given cell order (any
+   permutation, e.g. from unap_hip_rcm_order): cells renumbered, each face
+   turned so that its owner has the lower new label (its upper and lower
+   coefficients swap), faces stably sorted by owner, then neighbour.
+   new_face_of_old[f]: old face f's new position, or ~position when it turned
+   around (rcmLDU_nowrite's postEdgeOrder).  lower NULL: symmetric (lower is
+   upper).  Every output may be NULL. */
+int unap_hip_ldu_renumber(int64_t n, int64_t n_faces, const int64_t* lower_addr, const int64_t* upper_addr,
+                          const int64_t* new_of_old, const double* lower, const double* diag, const double* upper,
+                          int64_t* lower_addr_out, int64_t* upper_addr_out, int64_t* new_face_of_old,
+                          double* lower_out, double* diag_out, double* upper_out);
 
 /* ------------------------------------------------------------------ batches
  * Many independent LDU systems solved in one persistent kernel launch: one

@@ -100,6 +100,8 @@ class CpuSolvePath:
         _sig(L, P + "read_vector_openfoam", None, [ctypes.c_char_p, ctypes.c_int64, _f64p])
         _sig(L, P + "read_hypre", _vp, [ctypes.c_char_p])
         _sig(L, P + "read_vector_hypre", None, [ctypes.c_char_p, ctypes.c_int64, _f64p])
+        if P == "ur_":
+            _sig(L, P + "rcm_ldu", None, [ctypes.c_int64, ctypes.c_int64, _i64p, _i64p, _i64p, _i64p])
         self._tools_bound = True
 
     def ldu_arrays(self, h):

@@ -26,6 +26,7 @@
 #include "matrixConversion.hpp"
 #include "printUNAP.hpp"
 #include "readFromHypre.hpp"
+#include "rcmf.h"
 #include "readFromOpenFOAM.hpp"
 #include "unapMPI.hpp"
 
@@ -567,6 +568,15 @@ void ur_read_vector_hypre(const char* path, int64_t n, double* out) {
     scalarField b(n);
     constructVectorFromHypre(b, path);
     for (label i = 0; i < n; ++i) out[i] = b[i];
+}
+
+// rcmLDU_nowrite (cooRCM.cpp:477-637): the new label of every cell and the
+// new position of every face (negative-encoded where the face turned).  Its
+// cost grows with the square of the face count: small graphs only.
+void ur_rcm_ldu(int64_t nf, int64_t n, const int64_t* l, const int64_t* u, int64_t* newOfOld,
+                int64_t* newFaceOfOld) {
+    rcmLDU_nowrite(nf, n, reinterpret_cast<const label*>(l), reinterpret_cast<const label*>(u),
+                   reinterpret_cast<label*>(newOfOld), reinterpret_cast<label*>(newFaceOfOld));
 }
 
 }  // extern "C"

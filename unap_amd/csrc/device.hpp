@@ -208,6 +208,7 @@ struct DevMatrix {
 
     // Values from host coefficients (external order).
     void build(const HostLdu& A, const Layout& L, hipStream_t s) {
+        StageTimer t("device matrix: upload + values from the host", L.n);
         buildStructure(L, s);
         std::vector<T> v;
         fillSellValues<T>(L, A, false, v);
@@ -220,6 +221,7 @@ struct DevMatrix {
     // Values from device coefficients in external order (a coarse level's
     // Galerkin product); the slot map stays on the device for later refills.
     void buildOnDevice(const Layout& L, const double* d, const double* up, const double* lo, hipStream_t s) {
+        StageTimer t("device matrix: upload + values on the device", L.n);
         buildStructure(L, s);
         src.upload(L.src, s);
         nnzPadded = L.nnzPadded;

@@ -167,6 +167,76 @@ int guard(F&& f) {
     }
 }
 
+// Faces of every cell in ascending face id (the order the reference's
+// doubled COO lists a row's entries in, cooRCM.cpp:484-524).
+struct CellFaces {
+    std::vector<int64_t> start, face;
+};
+CellFaces cellFaces(int64_t n, int64_t nf, const int64_t* l, const int64_t* u) {
+    CellFaces C;
+    C.start.assign(n + 1, 0);
+    for (int64_t f = 0; f < nf; ++f) {
+        if (l[f] < 0 || l[f] >= n || u[f] < 0 || u[f] >= n || l[f] == u[f]) fail("face address out of range");
+        C.start[l[f] + 1]++;
+        C.start[u[f] + 1]++;
+    }
+    for (int64_t c = 0; c < n; ++c) C.start[c + 1] += C.start[c];
+    C.face.resize(2 * nf);
+    std::vector<int64_t> at(C.start.begin(), C.start.end() - 1);
+    for (int64_t f = 0; f < nf; ++f) {
+        C.face[at[l[f]]++] = f;
+        C.face[at[u[f]]++] = f;
+    }
+    return C;
+}
+
+// Reverse Cuthill-McKee (ReorderingSSM, cooRCM.cpp:79-235): each connected
+// component starts at its unvisited cell of least degree (the lowest id among
+// equals); a visited cell's unvisited neighbours join the queue by ascending
+// degree, equal degrees in the cell's face order; the sequence is reversed.
+// Returns new_of_old (the new label of every cell).  The reference sorts the
+// neighbours with a comparator that answers "degrees differ", which is no
+// ordering, and looks the component seed up by cell id in a shrunken list:
+// its order is defined only where neither matters; here the ordering is the
+// one its comments describe.
+std::vector<int64_t> rcmOrder(int64_t n, int64_t nf, const int64_t* l, const int64_t* u) {
+    const CellFaces C = cellFaces(n, nf, l, u);
+    auto deg = [&](int64_t c) { return C.start[c + 1] - C.start[c]; };
+    // cells by (degree, id): the seed of each component is the first unvisited one
+    std::vector<int64_t> byDeg(n);
+    for (int64_t c = 0; c < n; ++c) byDeg[c] = c;
+    std::stable_sort(byDeg.begin(), byDeg.end(), [&](int64_t a, int64_t b) { return deg(a) < deg(b); });
+    std::vector<char> seen(n, 0);
+    std::vector<int64_t> cm;
+    cm.reserve(n);
+    std::vector<int64_t> nb;
+    size_t seedAt = 0;
+    while ((int64_t)cm.size() < n) {
+        while (seen[byDeg[seedAt]]) ++seedAt;
+        const int64_t s0 = byDeg[seedAt];
+        seen[s0] = 1;
+        size_t head = cm.size();
+        cm.push_back(s0);
+        for (; head < cm.size(); ++head) {
+            const int64_t c = cm[head];
+            nb.clear();
+            for (int64_t k = C.start[c]; k < C.start[c + 1]; ++k) {
+                const int64_t f = C.face[k];
+                const int64_t o = l[f] == c ? u[f] : l[f];
+                if (!seen[o]) {
+                    seen[o] = 1;
+                    nb.push_back(o);
+                }
+            }
+            std::stable_sort(nb.begin(), nb.end(), [&](int64_t a, int64_t b) { return deg(a) < deg(b); });
+            cm.insert(cm.end(), nb.begin(), nb.end());
+        }
+    }
+    std::vector<int64_t> newOfOld(n);
+    for (int64_t k = 0; k < n; ++k) newOfOld[cm[n - 1 - k]] = k;
+    return newOfOld;
+}
+
 }  // namespace
 
 extern "C" {
@@ -349,6 +419,69 @@ int unap_hip_vector_read_hypre(const char* path, int64_t n, double* out) {
         if ((int64_t)lines.size() != m + 1) fail("vector file: line count does not match the row range");
         if (m != n) fail("vector file holds " + std::to_string(m) + " values, " + std::to_string(n) + " expected");
         for (int64_t i = 0; i < n; ++i) out[i] = toScalar(tok(tokens(lines[1 + i]), 1, "value"));
+    });
+}
+
+// ReverseCuthillMckee over the LDU graph (rcmLDU_nowrite, cooRCM.cpp:477-637).
+int unap_hip_rcm_order(int64_t n, int64_t n_faces, const int64_t* lower_addr, const int64_t* upper_addr,
+                       int64_t* new_of_old) {
+    return guard([&] {
+        if (n <= 0 || n >= (1LL << 31) || n_faces < 0) fail("sizes out of range");
+        if (!new_of_old || (n_faces > 0 && !(lower_addr && upper_addr))) fail("null argument");
+        const std::vector<int64_t> r = rcmOrder(n, n_faces, lower_addr, upper_addr);
+        std::memcpy(new_of_old, r.data(), sizeof(int64_t) * n);
+    });
+}
+
+// rcmLDU_rewrite (cooRCM.cpp:639-785) for a given cell order: cells
+// renumbered, each face turned so that its owner has the lower new label
+// (upper and lower coefficients swap), faces stably sorted by owner, then
+// neighbour.  new_face_of_old: rcmLDU_nowrite's postEdgeOrder, each old
+// face's new position, ~position when it turned around.
+int unap_hip_ldu_renumber(int64_t n, int64_t n_faces, const int64_t* lower_addr, const int64_t* upper_addr,
+                          const int64_t* new_of_old, const double* lower, const double* diag, const double* upper,
+                          int64_t* lower_addr_out, int64_t* upper_addr_out, int64_t* new_face_of_old,
+                          double* lower_out, double* diag_out, double* upper_out) {
+    return guard([&] {
+        if (n <= 0 || n >= (1LL << 31) || n_faces < 0) fail("sizes out of range");
+        if (!new_of_old || (n_faces > 0 && !(lower_addr && upper_addr))) fail("null argument");
+        std::vector<char> hit(n, 0);
+        for (int64_t c = 0; c < n; ++c) {
+            const int64_t v = new_of_old[c];
+            if (v < 0 || v >= n || hit[v]) fail("new_of_old is not a permutation of the cells");
+            hit[v] = 1;
+        }
+        std::vector<int64_t> ol(n_faces), ou(n_faces), order(n_faces);
+        std::vector<char> turned(n_faces, 0);
+        for (int64_t f = 0; f < n_faces; ++f) {
+            if (lower_addr[f] < 0 || lower_addr[f] >= n || upper_addr[f] < 0 || upper_addr[f] >= n ||
+                lower_addr[f] == upper_addr[f])
+                fail("face address out of range");
+            int64_t a = new_of_old[lower_addr[f]], b = new_of_old[upper_addr[f]];
+            if (a > b) {
+                std::swap(a, b);
+                turned[f] = 1;
+            }
+            ol[f] = a;
+            ou[f] = b;
+            order[f] = f;
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int64_t x, int64_t y) { return ol[x] != ol[y] ? ol[x] < ol[y] : ou[x] < ou[y]; });
+        if ((upper_out || lower_out) && n_faces > 0 && !upper) fail("coefficient output without upper");
+        const double* lw = lower ? lower : upper;  // symmetric: lower is upper
+        for (int64_t k = 0; k < n_faces; ++k) {
+            const int64_t f = order[k];
+            if (lower_addr_out) lower_addr_out[k] = ol[f];
+            if (upper_addr_out) upper_addr_out[k] = ou[f];
+            if (new_face_of_old) new_face_of_old[f] = turned[f] ? ~k : k;
+            if (upper_out) upper_out[k] = turned[f] ? lw[f] : upper[f];
+            if (lower_out) lower_out[k] = turned[f] ? upper[f] : lw[f];
+        }
+        if (diag_out) {
+            if (!diag) fail("diag_out without diag");
+            for (int64_t c = 0; c < n; ++c) diag_out[new_of_old[c]] = diag[c];
+        }
     });
 }
 

@@ -366,15 +366,24 @@ Layout buildLayout(const HostLdu& A, int orderMode, int sigma) {
     if (sigma < 0) sigma = kSigma;
     const int forced = forcedOrderMode();
     if (forced >= 0) orderMode = forced;
-    const RowFaces R = rowFaces(A);
+    StageTimer all("layout (all)", A.n);
+    RowFaces R;
+    {
+        StageTimer t("layout: row faces", A.n);
+        R = rowFaces(A);
+    }
     const bool sweepable = A.ownerSorted();
     std::vector<int> lev, blev;
     int nlev, nblev;
-    levels(A, R, sweepable, lev, nlev, blev, nblev);
+    {
+        StageTimer t("layout: sweep levels", A.n);
+        levels(A, R, sweepable, lev, nlev, blev, nblev);
+    }
     std::vector<int> perm(A.n);
     if (orderMode == 2) {
         std::iota(perm.begin(), perm.end(), 0);  // the caller's order
     } else {
+        StageTimer t("layout: breadth-first order + windows", A.n);
         const std::vector<int> bfs = bfsOrder(A, R);
         if (orderMode == 1) {
             perm = bfs;
@@ -388,6 +397,7 @@ Layout buildLayout(const HostLdu& A, int orderMode, int sigma) {
             for (int k = 0; k < nlev; ++k) sortWindows(perm, R, start[k], start[k + 1], sigma);
         }
     }
+    StageTimer t("layout: schedules + SELL assembly", A.n);
     return finishLayout(A, R, std::move(perm), lev, nlev, blev, nblev, sweepable);
 }
 

@@ -10,11 +10,34 @@
 // then the owner-side terms (src/matrix/lduMatrix/lduMatrixSpMV.cpp:87-91).
 #pragma once
 
+#include <chrono>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
 namespace unap_hip {
+
+// Setup stage times on stderr ("unap_hip setup: <stage> <ms>") when the
+// environment variable UNAP_HIP_SETUP_TIMES is set: a diagnostic for where
+// host and device setup time goes.
+inline bool setupTimesOn() {
+    static const bool on = std::getenv("UNAP_HIP_SETUP_TIMES") != nullptr;
+    return on;
+}
+struct StageTimer {
+    const char* what;
+    long long size;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    StageTimer(const char* w, long long n) : what(w), size(n) {}
+    ~StageTimer() {
+        if (!setupTimesOn()) return;
+        const double ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        std::fprintf(stderr, "unap_hip setup: %s n=%lld %.1f ms\n", what, size, ms);
+    }
+};
 
 // SELL slice height: slice s holds internal rows 32s .. 32s+31, entry k of a
 // row at sliceStart[s] + 32k + (row & 31).  A 64-lane wavefront covers two

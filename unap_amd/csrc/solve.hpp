@@ -37,7 +37,7 @@ struct LduView {
 };
 struct LduObj {
     HostLdu host;
-    int nFwdLevels = 0, nBwdLevels = 0;
+    int nFwdLevels = -1, nBwdLevels = -1;  // sweep level counts; -1 until asked for
     // bumped by every coefficient update: hierarchies built on this matrix
     // recompute their Galerkin levels when it moves
     long long coeffGen = 0;
@@ -286,7 +286,10 @@ struct AgglHost {
 };
 inline void buildAggl(const HostLdu& fine, const std::vector<double>& w, int maxLevels, int nCoarsest,
                       AgglHost& out) {
-    out.levels = agglomerate(fine, w, maxLevels, nCoarsest);
+    {
+        StageTimer t("agglomeration (host, every level)", fine.n);
+        out.levels = agglomerate(fine, w, maxLevels, nCoarsest);
+    }
     for (auto& v : out.layouts) v.clear();
     for (auto& v : out.built) v.clear();
     out.dev.clear();

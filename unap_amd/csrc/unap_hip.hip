@@ -230,6 +230,7 @@ void Hierarchy<T>::refresh() {
 template <typename T>
 void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgParams& p) {
     Ctx& c = Ctx::get();
+    StageTimer total("hierarchy (all, after the agglomeration)", A.host.n);
     fineObj = &A;
     order = (p.smoother == UNAP_HIP_SMOOTHER_GS) ? ORDER_SWEEP : ORDER_STREAM;
     fine = &Fine<T>::get(A, order);
@@ -247,7 +248,10 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     const bool fp32 = sizeof(T) == 4;
     const bool fixed = prm.fixedCycle;
     if (fp32 && !fixed) throw Error(UNAP_HIP_ERR_ARG, "the FP32 hierarchy needs the fixed cycle");
-    if (ag->valuesGen != A.coeffGen) ag->galerkin(A.host, A.coeffGen);
+    if (ag->valuesGen != A.coeffGen) {
+        StageTimer t("hierarchy: Galerkin (device, with list setup)", A.host.n);
+        ag->galerkin(A.host, A.coeffGen);
+    }
     mats.clear();
     cs.clear();
     ch.clear();
@@ -264,7 +268,11 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
         std::vector<int> rm = ag->levels[k0].restrictMap;
         for (int k = k0 + 1; k <= sel[l]; ++k)
             for (int& v : rm) v = ag->levels[k].restrictMap[v];
-        Transfer tr = buildTransfer(fl, cl, rm);
+        Transfer tr;
+        {
+            StageTimer tt("hierarchy: transfer maps", (long long)rm.size());
+            tr = buildTransfer(fl, cl, rm);
+        }
         cs.emplace_back();
         ch.emplace_back();
         f2c.emplace_back();
@@ -290,7 +298,10 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     firstCall.assign(L, 1);
     est.alloc(L);
     mgst.alloc(1);
-    setupNumeric();
+    {
+        StageTimer t("hierarchy: numeric setup (coarsest level, estimates)", A.host.n);
+        setupNumeric();
+    }
     valuesGen = A.coeffGen;
 }
 
@@ -1005,7 +1016,6 @@ int unap_hip_ldu_create(int64_t n, int64_t nf, const int64_t* la, const int64_t*
         A.up.assign(upper, upper + nf);
         A.symm = (lower == nullptr || lower == upper);
         if (!A.symm) A.lo.assign(lower, lower + nf);
-        levelCounts(A, h->o.nFwdLevels, h->o.nBwdLevels);
         *out = h;
     });
 }
@@ -1020,6 +1030,7 @@ int unap_hip_ldu_info(unap_hip_ldu A, int64_t* n, int64_t* nf, int* symm, int64_
         if (n) *n = A->o.host.n;
         if (nf) *nf = A->o.host.nFaces();
         if (symm) *symm = A->o.host.symm ? 1 : 0;
+        if ((fl || bl) && A->o.nFwdLevels < 0) levelCounts(A->o.host, A->o.nFwdLevels, A->o.nBwdLevels);
         if (fl) *fl = A->o.nFwdLevels;
         if (bl) *bl = A->o.nBwdLevels;
     });
@@ -1096,7 +1107,7 @@ int unap_hip_ldu_set_topology(unap_hip_ldu A, int64_t nf, const int64_t* la, con
         o.agglCache.reset();
         o.topoGen += 1;
         o.coeffGen += 1;
-        levelCounts(H, o.nFwdLevels, o.nBwdLevels);
+        o.nFwdLevels = o.nBwdLevels = -1;
     });
 }
 

@@ -69,6 +69,9 @@ SIGNATURES = {
     "unap_hip_vector_write_openfoam": (ctypes.c_int, [ctypes.c_char_p, _i64, _f64p]),
     "unap_hip_ldu_read_hypre": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "unap_hip_vector_read_hypre": (ctypes.c_int, [ctypes.c_char_p, _i64, _f64p]),
+    "unap_hip_rcm_order": (ctypes.c_int, [_i64, _i64, _i64p, _i64p, _i64p]),
+    "unap_hip_ldu_renumber": (ctypes.c_int, [_i64, _i64, _i64p, _i64p, _i64p, _f64p, _f64p, _f64p,
+                                             _i64p, _i64p, _i64p, _f64p, _f64p, _f64p]),
     "unap_hip_ldu_spmv": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "unap_hip_ldu_spmv_noalloc": (ctypes.c_int, [_vp, _vp, _vp]),
     "unap_hip_amg_params_default": (ctypes.c_int, [ctypes.POINTER(AmgParams)]),
@@ -279,6 +282,37 @@ def read_vector_hypre(path, n):
     return out
 
 
+def rcm_order(n, lower_addr, upper_addr):
+    """Reverse Cuthill-McKee order (cooRCM.cpp): the new label of every cell."""
+    l = np.ascontiguousarray(lower_addr, np.int64)
+    u = np.ascontiguousarray(upper_addr, np.int64)
+    out = np.empty(n, np.int64)
+    _check(lib().unap_hip_rcm_order(n, l.size, _p(l, _i64p), _p(u, _i64p), _p(out, _i64p)))
+    return out
+
+
+def renumber(n, lower_addr, upper_addr, new_of_old, diag=None, upper=None, lower=None):
+    """rcmLDU_rewrite for a given cell order: (lowerAddr, upperAddr,
+    new_face_of_old, lower, diag, upper) of the renumbered matrix (faces
+    turned to owner < neighbour and sorted; ~k marks a face that turned;
+    lower None for a symmetric matrix)."""
+    l = np.ascontiguousarray(lower_addr, np.int64)
+    u = np.ascontiguousarray(upper_addr, np.int64)
+    p = np.ascontiguousarray(new_of_old, np.int64)
+    nf = l.size
+    d = None if diag is None else np.ascontiguousarray(diag, np.float64)
+    up = None if upper is None else np.ascontiguousarray(upper, np.float64)
+    lo = None if lower is None else np.ascontiguousarray(lower, np.float64)
+    lo_out = np.empty(nf) if (up is not None and lo is not None) else None
+    up_out = np.empty(nf) if up is not None else None
+    d_out = np.empty(n) if d is not None else None
+    l_out, u_out, fmap = np.empty(nf, np.int64), np.empty(nf, np.int64), np.empty(nf, np.int64)
+    _check(lib().unap_hip_ldu_renumber(n, nf, _p(l, _i64p), _p(u, _i64p), _p(p, _i64p), _p(lo), _p(d), _p(up),
+                                       _p(l_out, _i64p), _p(u_out, _i64p), _p(fmap, _i64p), _p(lo_out), _p(d_out),
+                                       _p(up_out)))
+    return l_out, u_out, fmap, lo_out, d_out, up_out
+
+
 def dense_inverse_spd(a):
     """Inverse of a dense SPD matrix by the GPU's blocked Gauss-Jordan."""
     a = np.ascontiguousarray(a, np.float64)
@@ -338,7 +372,7 @@ class LduMatrix:
 
     def arrays(self):
         """(lowerAddr, upperAddr, lower, diag, upper) host copies; lower is None when symmetric."""
-        info = self.info()
+        info = self.info(levels=False)
         return self._arrays(info["nfaces"], info["symmetric"])
 
     @classmethod
@@ -383,13 +417,17 @@ class LduMatrix:
             _lib.unap_hip_ldu_destroy(self.h)
             self.h = None
 
-    def info(self):
+    def info(self, levels=True):
+        """Sizes and symmetry; with levels, the forward / backward sweep level
+        counts too (computed on the first request: a pass over every row)."""
         n, nf, fl, bl = _i64(), _i64(), _i64(), _i64()
         sy = ctypes.c_int()
-        _check(lib().unap_hip_ldu_info(self.h, ctypes.byref(n), ctypes.byref(nf), ctypes.byref(sy),
-                                       ctypes.byref(fl), ctypes.byref(bl)))
-        return {"n": n.value, "nfaces": nf.value, "symmetric": bool(sy.value),
-                "fwd_levels": fl.value, "bwd_levels": bl.value}
+        lv = (ctypes.byref(fl), ctypes.byref(bl)) if levels else (None, None)
+        _check(lib().unap_hip_ldu_info(self.h, ctypes.byref(n), ctypes.byref(nf), ctypes.byref(sy), *lv))
+        out = {"n": n.value, "nfaces": nf.value, "symmetric": bool(sy.value)}
+        if levels:
+            out.update(fwd_levels=fl.value, bwd_levels=bl.value)
+        return out
 
     def set_coeffs(self, diag, upper, lower=None):
         """setMatrixCoeffients: new values, same addressing (lower None: symmetric)."""
