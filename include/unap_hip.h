@@ -286,6 +286,12 @@ int unap_hip_solver_last_ms(unap_hip_solver S, double* ms);
 #define UNAP_HIP_KERNEL_COPY 8       /* y = x, a plain copy (achievable-bandwidth yardstick) */
 int unap_hip_profile_kernel(unap_hip_ldu A, int kernel, int precision, int64_t reps, double* avg_ms,
                             double* bytes);
+/* No counterpart: the setup built on the device (cell orders, sweep
+   schedules, SELL-32 layout of each order mode; the agglomeration's pairing
+   lists) against the same structures built on the host, for this matrix.
+   weights: agglomeration face weights (NULL: |upper|).  *mismatches: the
+   number of fields that differ (0: identical). */
+int unap_hip_setup_self_check(unap_hip_ldu A, const double* weights, int64_t* mismatches);
 
 /* ------------------------------------------------------- formats and files
  * Host-side construction of matrices from other formats and text files

@@ -132,6 +132,26 @@ void launchReduce(const Body& b, int n, const int* order, const Fin& fin, Skip s
     UH_LAUNCHED();
 }
 
+// A device-built layout's column and slot-source arrays, kept in HBM.
+struct LayoutDev {
+    DBuf<int> col, src;
+};
+// The layout with col and src on the host too (downloaded where they live in
+// HBM only), for host code that walks a small matrix's slots.
+inline Layout hostArrays(const Layout& L, hipStream_t s) {
+    Layout H = L;
+    if (!L.dev) return H;
+    H.col.resize(L.nnzPadded);
+    H.src.resize(L.nnzPadded);
+    if (L.nnzPadded > 0) {
+        UH_CHECK(hipMemcpyAsync(H.col.data(), L.dev->col.get(), sizeof(int) * L.nnzPadded, hipMemcpyDeviceToHost, s));
+        UH_CHECK(hipMemcpyAsync(H.src.data(), L.dev->src.get(), sizeof(int) * L.nnzPadded, hipMemcpyDeviceToHost, s));
+    }
+    UH_CHECK(hipStreamSynchronize(s));
+    H.dev.reset();
+    return H;
+}
+
 // The backward sweep's entries in backward-level order: position q (row
 // bwdRows[q]) holds that row's owner-side entries from the last down to the
 // first, the order the sweep consumes them, in SELL slices over positions.
@@ -210,20 +230,57 @@ struct DevMatrix {
     void build(const HostLdu& A, const Layout& L, hipStream_t s) {
         StageTimer t("device matrix: upload + values from the host", L.n);
         buildStructure(L, s);
-        std::vector<T> v;
-        fillSellValues<T>(L, A, false, v);
-        val.upload(v, s);
-        std::vector<T> d;
-        fillDiag<T>(L, A, d);
-        diag.upload(d, s);
+        fillValues(A, L, false, val, s);
+        fillDiagFrom(A, s);
         UH_CHECK(hipStreamSynchronize(s));  // host staging vectors die here
+    }
+    // SELL values from host coefficients: on the device from the face arrays
+    // where the layout's slot sources are in HBM, else assembled on the host
+    // (the same values either way).  upperBoth: upper on both sides.
+    void fillValues(const HostLdu& A, const Layout& L, bool upperBoth, DBuf<T>& out, hipStream_t s) {
+        if (!L.dev) {
+            std::vector<T> v;
+            fillSellValues<T>(L, A, upperBoth, v);
+            out.upload(v, s);
+            UH_CHECK(hipStreamSynchronize(s));
+            return;
+        }
+        DBuf<double> up, lo;
+        up.upload(A.up, s);
+        if (!A.symm && !upperBoth) lo.upload(A.lo, s);
+        out.alloc(std::max<long long>(L.nnzPadded, 1));
+        if (L.nnzPadded > 0) {
+            const long long g = std::min<long long>((L.nnzPadded + 255) / 256, 8192);
+            kFillSell<T><<<(unsigned)g, 256, 0, s>>>(L.nnzPadded, L.dev->src.get(), up.get(), lo.p ? lo.get() : nullptr,
+                                                     upperBoth ? 1 : 0, out.get());
+            UH_LAUNCHED();
+        }
+        UH_CHECK(hipStreamSynchronize(s));
+    }
+    // diag in internal order, gathered on the device through perm
+    void fillDiagFrom(const HostLdu& A, hipStream_t s) {
+        DBuf<double> d;
+        d.upload(A.d, s);
+        diag.alloc(std::max(n, 1));
+        if (n > 0) {
+            kFillDiag<T><<<(n + 255) / 256, 256, 0, s>>>(n, perm.get(), d.get(), diag.get());
+            UH_LAUNCHED();
+        }
+        UH_CHECK(hipStreamSynchronize(s));
     }
     // Values from device coefficients in external order (a coarse level's
     // Galerkin product); the slot map stays on the device for later refills.
     void buildOnDevice(const Layout& L, const double* d, const double* up, const double* lo, hipStream_t s) {
         StageTimer t("device matrix: upload + values on the device", L.n);
         buildStructure(L, s);
-        src.upload(L.src, s);
+        if (L.dev) {
+            src.alloc(std::max<long long>(L.nnzPadded, 1));
+            if (L.nnzPadded > 0)
+                UH_CHECK(hipMemcpyAsync(src.get(), L.dev->src.get(), sizeof(int) * L.nnzPadded,
+                                        hipMemcpyDeviceToDevice, s));
+        } else {
+            src.upload(L.src, s);
+        }
         nnzPadded = L.nnzPadded;
         fillOnDevice(d, up, lo, s);
         UH_CHECK(hipStreamSynchronize(s));
@@ -254,7 +311,14 @@ struct DevMatrix {
         maxRowLen = L.maxRowLen;
         sliceStart.upload(L.sliceStart, s);
         rowInfo.upload(L.rowInfo, s);
-        col.upload(L.col, s);
+        if (L.dev) {
+            col.alloc(std::max<long long>(L.nnzPadded, 1));
+            if (L.nnzPadded > 0)
+                UH_CHECK(hipMemcpyAsync(col.get(), L.dev->col.get(), sizeof(int) * L.nnzPadded,
+                                        hipMemcpyDeviceToDevice, s));
+        } else {
+            col.upload(L.col, s);
+        }
         perm.upload(L.perm, s);
         iperm.upload(L.iperm, s);
         fwdStartH = L.fwdStart;
@@ -280,20 +344,12 @@ struct DevMatrix {
     }
     void buildValU(const HostLdu& A, const Layout& L, hipStream_t s) {
         if (A.symm || valU.p) return;
-        std::vector<T> v;
-        fillSellValues<T>(L, A, true, v);
-        valU.upload(v, s);
-        UH_CHECK(hipStreamSynchronize(s));
+        fillValues(A, L, true, valU, s);
     }
     void updateValues(const HostLdu& A, const Layout& L, hipStream_t s) {
         const bool hadU = valU.p != nullptr;
-        std::vector<T> v;
-        fillSellValues<T>(L, A, false, v);
-        val.upload(v, s);
-        std::vector<T> d;
-        fillDiag<T>(L, A, d);
-        diag.upload(d, s);
-        UH_CHECK(hipStreamSynchronize(s));
+        fillValues(A, L, false, val, s);
+        fillDiagFrom(A, s);
         valU.release();
         bwdVal.release();  // rebuilt from the new values on the next backward sweep
         if (hadU) buildValU(A, L, s);

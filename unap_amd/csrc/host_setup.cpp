@@ -15,27 +15,6 @@ namespace unap_hip {
 
 static const double kSmall = 1.0e-37;  // SMALL, src/base/unap.hpp:88
 
-namespace {
-// f(begin, end) over [0, n) split among up to 16 host threads: for loops
-// whose iterations write disjoint elements, so the result does not depend
-// on the split.  Small ranges run on the calling thread.
-template <class F>
-void parallelFor(long long n, F&& f) {
-    const int T = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
-    if (T == 1 || n < (1 << 16)) {
-        f(0LL, n);
-        return;
-    }
-    const long long chunk = (n + T - 1) / T;
-    std::vector<std::thread> th;
-    for (int t = 0; t < T; ++t) {
-        const long long a = t * chunk, b = std::min(n, a + chunk);
-        if (a < b) th.emplace_back([&f, a, b] { f(a, b); });
-    }
-    for (auto& x : th) x.join();
-}
-}  // namespace
-
 bool HostLdu::ownerSorted() const {
     const int nf = nFaces();
     for (int f = 0; f < nf; ++f) {
@@ -456,49 +435,70 @@ namespace {
 
 // Pairwise greedy clustering of one level (lduAgglomerate.cpp:94-271, cells
 // visited in ascending order).  Each cell scans its faces as the neighbour
-// first, then as the owner, each in ascending face id.
-std::vector<int> pairCells(const HostLdu& A, const std::vector<double>& w, int& nCoarse) {
-    const int n = A.n, nf = A.nFaces();
-    std::vector<int> off(n + 1, 0);
-    for (int f = 0; f < nf; ++f) {
-        off[A.u[f] + 1]++;
-        off[A.l[f] + 1]++;
+// first, then as the owner, each in ascending face id (the cellFaces order
+// of lduAgglomerate.cpp:114-152).  The lists are built on host threads and
+// hold, per entry, the cell across the face and the face weight, so the
+// sequential greedy pass reads them contiguously and looks up only the
+// cluster of each neighbour (prefetched a few cells ahead).  Of the face's
+// two cells one is the visited cell itself, whose cluster is still unset:
+// the reference's tests on both ends reduce to the test on the neighbour.
+std::vector<int> pairCells(const HostLdu& A, const std::vector<double>& w, int& nCoarse, const PairLists& lists) {
+    const int n = A.n;
+    std::vector<int> rs, nbr;
+    std::vector<double> wk;
+    {
+        StageTimer tl("agglomeration: pairing lists", n);
+        if (!(lists && lists(A, w, rs, nbr, wk))) {
+            RowFaces R = rowFaces(A);  // per cell, by face id: 2f (neighbour side), 2f + 1 (owner side)
+            rs = std::move(R.rs);
+            nbr.resize(R.ent.size());
+            wk.resize(R.ent.size());
+            parallelFor(n, [&](long long a, long long b) {
+                for (long long c = a; c < b; ++c) {
+                    int k = rs[c];
+                    for (int side = 0; side < 2; ++side)
+                        for (int j = rs[c]; j < rs[c + 1]; ++j) {
+                            const int e = R.ent[j];
+                            if ((e & 1) != side) continue;
+                            const int f = e >> 1;
+                            nbr[k] = side ? A.u[f] : A.l[f];
+                            wk[k] = w[f];
+                            ++k;
+                        }
+                }
+            });
+        }
     }
-    for (int c = 0; c < n; ++c) off[c + 1] += off[c];
-    std::vector<int> faces(2 * (size_t)nf), at(off.begin(), off.end() - 1);
-    for (int f = 0; f < nf; ++f) faces[at[A.u[f]]++] = f;
-    for (int f = 0; f < nf; ++f) faces[at[A.l[f]]++] = f;
-
+    StageTimer tg("agglomeration: pairing greedy pass", n);
     std::vector<int> map(n, -1);
     nCoarse = 0;
+    constexpr int kAhead = 8;
     for (int c = 0; c < n; ++c) {
+        if (c + kAhead < n)
+            for (int k = rs[c + kAhead]; k < rs[c + kAhead + 1]; ++k) __builtin_prefetch(&map[nbr[k]]);
         if (map[c] >= 0) continue;
         int best = -1;
         double bestW = kSmall;
-        for (int k = off[c]; k < off[c + 1]; ++k) {
-            const int f = faces[k];
-            if (map[A.u[f]] < 0 && map[A.l[f]] < 0 && w[f] > bestW) {
-                best = f;
-                bestW = w[f];
+        for (int k = rs[c]; k < rs[c + 1]; ++k)
+            if (map[nbr[k]] < 0 && wk[k] > bestW) {
+                best = k;
+                bestW = wk[k];
             }
-        }
         if (best >= 0) {
-            map[A.u[best]] = nCoarse;
-            map[A.l[best]] = nCoarse;
+            map[nbr[best]] = nCoarse;
+            map[c] = nCoarse;
             ++nCoarse;
             continue;
         }
         // no free neighbour: join the cluster across the heaviest face
         int join = -1;
         double joinW = kSmall;
-        for (int k = off[c]; k < off[c + 1]; ++k) {
-            const int f = faces[k];
-            if (w[f] > joinW) {
-                join = f;
-                joinW = w[f];
+        for (int k = rs[c]; k < rs[c + 1]; ++k)
+            if (wk[k] > joinW) {
+                join = k;
+                joinW = wk[k];
             }
-        }
-        if (join >= 0) map[c] = std::max(map[A.u[join]], map[A.l[join]]);
+        if (join >= 0) map[c] = std::max(map[c], map[nbr[join]]);
     }
     for (int c = 0; c < n; ++c)
         if (map[c] < 0) map[c] = nCoarse++;
@@ -507,17 +507,17 @@ std::vector<int> pairCells(const HostLdu& A, const std::vector<double>& w, int& 
 
 // Coarse faces from fine faces joining different clusters, numbered owner-major
 // with neighbours in first-appearance order (lduAgglomerationAddressing.cpp:57-178).
-// The fine faces are bucketed by coarse owner (a counting sort that keeps
-// ascending face order), then each owner's bucket is walked once: a coarse
-// neighbour met for the first time in this bucket gets the next coarse face
-// id, which is its owner-major, first-appearance position.
-void coarseAddressing(const HostLdu& F, CoarseLevel& C) {
+// The fine faces are grouped by coarse owner, ascending inside each group;
+// within a group a coarse neighbour met for the first time gets the group's
+// next coarse face id, which is its owner-major, first-appearance position.
+// Groups are independent, so they are numbered on host threads once their
+// sizes are known.  The next level's face weights (restrictFaceField,
+// unapMatrix.hpp:441-475) are summed per group in ascending fine face order,
+// the order of the reference's loop over the fine faces.
+void coarseAddressing(const HostLdu& F, CoarseLevel& C, const std::vector<double>& w, std::vector<double>& cw) {
     const int nf = F.nFaces();
     const int nc = C.nCoarse;
     C.faceRestrictMap.assign(nf, 0);
-    // each fine face's coarse owner and neighbour (-1: inside one cluster),
-    // looked up once, in parallel: the ordered passes below then read them
-    // sequentially
     std::vector<int> ownOf(nf), neiOf(nf);
     parallelFor(nf, [&](long long fa, long long fb) {
         for (long long f = fa; f < fb; ++f) {
@@ -531,47 +531,74 @@ void coarseAddressing(const HostLdu& F, CoarseLevel& C) {
             }
         }
     });
-    std::vector<int> start(nc + 1, 0);
-    for (int f = 0; f < nf; ++f)
-        if (ownOf[f] >= 0) start[ownOf[f] + 1]++;
-    for (int c = 0; c < nc; ++c) start[c + 1] += start[c];
-    std::vector<int> bucket(start[nc]), at(start.begin(), start.end() - 1);
-    for (int f = 0; f < nf; ++f)
-        if (ownOf[f] >= 0) bucket[at[ownOf[f]]++] = f;
-    std::vector<int> seenBy(nc, -1), slot(nc, 0);
-    C.A.n = nc;
-    C.A.l.clear();
-    C.A.u.clear();
-    for (int own = 0; own < nc; ++own)
-        for (int k = start[own]; k < start[own + 1]; ++k) {
-            const int f = bucket[k];
-            const int nei = neiOf[f];
-            if (seenBy[nei] != own) {
-                seenBy[nei] = own;
-                slot[nei] = (int)C.A.l.size();
-                C.A.l.push_back(own);
-                C.A.u.push_back(nei);
+    std::vector<int> start, bucket;
+    stableBuckets(nf, nc, [&](long long f) { return ownOf[f]; }, start, bucket);
+    // first appearance of each coarse neighbour inside a group
+    auto firstSeen = [&](int k, int own) {
+        const int nei = neiOf[bucket[k]];
+        for (int j = start[own]; j < k; ++j)
+            if (neiOf[bucket[j]] == nei) return j;
+        return k;
+    };
+    std::vector<int> first(bucket.size()), nNew(nc);
+    parallelFor(nc, [&](long long a, long long b) {
+        for (long long own = a; own < b; ++own) {
+            int m = 0;
+            for (int k = start[own]; k < start[own + 1]; ++k) {
+                first[k] = firstSeen(k, (int)own);
+                m += first[k] == k;
             }
-            C.faceRestrictMap[f] = slot[nei];
+            nNew[own] = m;
         }
+    });
+    std::vector<int> faceStart(nc + 1, 0);
+    for (int own = 0; own < nc; ++own) faceStart[own + 1] = faceStart[own] + nNew[own];
+    const int ncf = faceStart[nc];
+    C.A.n = nc;
+    C.A.l.assign(ncf, 0);
+    C.A.u.assign(ncf, 0);
+    cw.assign(ncf, 0.0);
+    std::vector<int> slotAt(bucket.size());
+    parallelFor(nc, [&](long long a, long long b) {
+        for (long long own = a; own < b; ++own) {
+            int next = faceStart[own];
+            for (int k = start[own]; k < start[own + 1]; ++k) {
+                const int f = bucket[k];
+                int slot;
+                if (first[k] == k) {
+                    slot = next++;
+                    C.A.l[slot] = (int)own;
+                    C.A.u[slot] = neiOf[f];
+                } else {
+                    slot = slotAt[first[k]];
+                }
+                slotAt[k] = slot;
+                C.faceRestrictMap[f] = slot;
+                cw[slot] += w[f];
+            }
+        }
+    });
 }
 
 }  // namespace
 
 std::vector<CoarseLevel> agglomerate(const HostLdu& fine, const std::vector<double>& weights,
-                                     int maxLevels, int nCellsInCoarsestLevel) {
+                                     int maxLevels, int nCellsInCoarsestLevel, const AgglHooks* hooks) {
     std::vector<CoarseLevel> levels;
     std::vector<double> w(weights);
     while ((int)levels.size() < maxLevels - 1) {
         const HostLdu& F = levels.empty() ? fine : levels.back().A;
         CoarseLevel C;
-        C.restrictMap = pairCells(F, w, C.nCoarse);
+        {
+            StageTimer t("agglomeration: pairing", F.n);
+            C.restrictMap = pairCells(F, w, C.nCoarse, hooks ? hooks->lists : PairLists());
+        }
         if (!(C.nCoarse >= nCellsInCoarsestLevel)) break;
-        coarseAddressing(F, C);
-        // face weights of the next level: restrictFaceField (unapMatrix.hpp:441-475)
-        std::vector<double> cw(C.A.nFaces(), 0.0);
-        for (int f = 0; f < F.nFaces(); ++f)
-            if (C.faceRestrictMap[f] >= 0) cw[C.faceRestrictMap[f]] += w[f];
+        std::vector<double> cw;
+        {
+            StageTimer t("agglomeration: coarse addressing, face weights", F.n);
+            if (!(hooks && hooks->addressing && hooks->addressing(F, C, w, cw))) coarseAddressing(F, C, w, cw);
+        }
         w.swap(cw);
         levels.push_back(std::move(C));
     }
@@ -580,17 +607,47 @@ std::vector<CoarseLevel> agglomerate(const HostLdu& fine, const std::vector<doub
     return levels;
 }
 
+void galerkinListsHost(const CoarseLevel& C, const std::vector<int>& fl, std::vector<int>& cst, std::vector<int>& cl,
+                       std::vector<int>& ist, std::vector<int>& il, std::vector<int>& fst, std::vector<int>& ff) {
+    const int nfc = (int)C.restrictMap.size(), nff = (int)C.faceRestrictMap.size();
+    const int nc = C.nCoarse, ncf = C.A.nFaces();
+    stableBuckets(nfc, nc, [&](long long i) { return C.restrictMap[i]; }, cst, cl);
+    stableBuckets(nff, nc, [&](long long f) {
+        const int cf = C.faceRestrictMap[f];
+        return cf < 0 ? -1 - cf : -1;
+    }, ist, il);
+    stableBuckets(nff, ncf, [&](long long f) { return C.faceRestrictMap[f]; }, fst, ff);
+    // orientation (lduAgglomerateMatrix.cpp:50-66): the fine owner lies in
+    // the coarse owner, or the sides swap (~f)
+    std::atomic<bool> bad{false};
+    parallelFor(ncf, [&](long long a, long long b) {
+        for (long long cf = a; cf < b; ++cf)
+            for (int k = fst[cf]; k < fst[cf + 1]; ++k) {
+                const int f = ff[k];
+                const int ownerCell = C.restrictMap[fl[f]];
+                if (C.A.l[cf] == ownerCell) continue;
+                if (C.A.u[cf] == ownerCell)
+                    ff[k] = ~f;
+                else
+                    bad = true;
+            }
+    });
+    if (bad) throw std::runtime_error("agglomeration: inconsistent coarse addressing");
+}
+
 Transfer buildTransfer(const Layout& fineL, const Layout& coarseL, const std::vector<int>& rm) {
     Transfer T;
     const int nf = fineL.n, nc = coarseL.n;
-    T.childStart.assign(nc + 1, 0);
-    for (int e = 0; e < nf; ++e) T.childStart[coarseL.iperm[rm[e]] + 1]++;
-    for (int c = 0; c < nc; ++c) T.childStart[c + 1] += T.childStart[c];
-    T.child.assign(nf, 0);
-    std::vector<int> at(T.childStart.begin(), T.childStart.end() - 1);
-    for (int e = 0; e < nf; ++e) T.child[at[coarseL.iperm[rm[e]]]++] = fineL.iperm[e];
+    // children of each coarse internal cell, ascending by fine external id,
+    // as fine internal ids
+    stableBuckets(nf, nc, [&](long long e) { return coarseL.iperm[rm[e]]; }, T.childStart, T.child);
+    parallelFor(nf, [&](long long a, long long b) {
+        for (long long k = a; k < b; ++k) T.child[k] = fineL.iperm[T.child[k]];
+    });
     T.fineToCoarse.assign(nf, 0);
-    for (int i = 0; i < nf; ++i) T.fineToCoarse[i] = coarseL.iperm[rm[fineL.perm[i]]];
+    parallelFor(nf, [&](long long a, long long b) {
+        for (long long i = a; i < b; ++i) T.fineToCoarse[i] = coarseL.iperm[rm[fineL.perm[i]]];
+    });
     return T;
 }
 

@@ -10,14 +10,72 @@
 // then the owner-side terms (src/matrix/lduMatrix/lduMatrixSpMV.cpp:87-91).
 #pragma once
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace unap_hip {
+
+// f(begin, end) over [0, n) split among up to 16 host threads: for loops
+// whose iterations write disjoint elements, so the result does not depend
+// on the split.  Small ranges run on the calling thread.
+template <class F>
+void parallelFor(long long n, F&& f) {
+    const int T = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    if (T == 1 || n < (1 << 16)) {
+        f(0LL, n);
+        return;
+    }
+    const long long chunk = (n + T - 1) / T;
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) {
+        const long long a = t * chunk, b = std::min(n, a + chunk);
+        if (a < b) th.emplace_back([&f, a, b] { f(a, b); });
+    }
+    for (auto& x : th) x.join();
+}
+
+// Items 0..m-1 grouped by key(i) in [0, nb) (key -1: left out), ascending
+// inside each group: the result of a sequential counting sort over ascending
+// i, built on host threads (counts and placements by atomics, then each
+// group sorted).  start gets nb + 1 group boundaries.
+template <class Key>
+void stableBuckets(long long m, int nb, Key key, std::vector<int>& start, std::vector<int>& items) {
+    std::vector<std::atomic<int>> cnt(nb + 1);
+    parallelFor(nb + 1, [&](long long a, long long b) {
+        for (long long i = a; i < b; ++i) cnt[i].store(0, std::memory_order_relaxed);
+    });
+    parallelFor(m, [&](long long a, long long b) {
+        for (long long i = a; i < b; ++i) {
+            const int k = key(i);
+            if (k >= 0) cnt[k + 1].fetch_add(1, std::memory_order_relaxed);
+        }
+    });
+    start.assign(nb + 1, 0);
+    for (int k = 0; k < nb; ++k) start[k + 1] = start[k] + cnt[k + 1].load(std::memory_order_relaxed);
+    parallelFor(nb, [&](long long a, long long b) {
+        for (long long k = a; k < b; ++k) cnt[k].store(start[k], std::memory_order_relaxed);
+    });
+    items.assign(start[nb], 0);
+    parallelFor(m, [&](long long a, long long b) {
+        for (long long i = a; i < b; ++i) {
+            const int k = key(i);
+            if (k >= 0) items[cnt[k].fetch_add(1, std::memory_order_relaxed)] = (int)i;
+        }
+    });
+    parallelFor(nb, [&](long long a, long long b) {
+        for (long long k = a; k < b; ++k)
+            if (start[k + 1] - start[k] > 1) std::sort(items.begin() + start[k], items.begin() + start[k + 1]);
+    });
+}
 
 // Setup stage times on stderr ("unap_hip setup: <stage> <ms>") when the
 // environment variable UNAP_HIP_SETUP_TIMES is set: a diagnostic for where
@@ -66,6 +124,8 @@ struct HostLdu {
     bool ownerSorted() const;
 };
 
+struct LayoutDev;  // device.hpp
+
 // Device storage plan of one matrix.
 struct Layout {
     int n = 0, nf = 0, nslices = 0;
@@ -83,6 +143,9 @@ struct Layout {
     std::vector<int> bwdStart;          // backward levels: bwdRows[bwdStart[k] .. bwdStart[k+1])
     int maxRowLen = 0;
     bool sweepable = false;             // ownerSorted(): sweeps reproduce the reference
+    // layouts built on the device keep col and src in HBM only (the host
+    // vectors above then stay empty)
+    std::shared_ptr<LayoutDev> dev;
 };
 
 // orderMode 0: forward-level major, breadth-first within a level (sweeps
@@ -126,8 +189,31 @@ struct CoarseLevel {
 // step (:94-271) and coarse addressing (lduAgglomerationAddressing.cpp:8-178).
 // The Galerkin coefficients (lduAgglomerateMatrix.cpp:3-104) are not formed
 // here (galerkin.hpp).
+// lists (optional): builds a level's pairing lists -- per cell (row starts
+// rs), the cell across each face and the face weight, neighbour side first,
+// then owner side, each by face id -- and returns true, or returns false to
+// leave them to the host.
+using PairLists = std::function<bool(const HostLdu&, const std::vector<double>&, std::vector<int>& rs,
+                                     std::vector<int>& nbr, std::vector<double>& wk)>;
+// addressing (optional): a level's coarse faces, face map and next-level
+// face weights (coarseAddressing), returning true, or false to leave them
+// to the host.
+using CoarseAddr =
+    std::function<bool(const HostLdu&, CoarseLevel&, const std::vector<double>& w, std::vector<double>& cw)>;
+struct AgglHooks {
+    PairLists lists;
+    CoarseAddr addressing;
+};
 std::vector<CoarseLevel> agglomerate(const HostLdu& fine, const std::vector<double>& weights,
-                                     int maxLevels, int nCellsInCoarsestLevel);
+                                     int maxLevels, int nCellsInCoarsestLevel, const AgglHooks* hooks = nullptr);
+
+// Galerkin segment lists of level C on the host: children of each coarse
+// cell ascending by fine cell, inner faces ascending, fine faces of each
+// coarse face ascending (~f where the sides swap).  fineL: the fine level's
+// lowerAddr.
+void galerkinListsHost(const CoarseLevel& C, const std::vector<int>& fineL, std::vector<int>& cst,
+                       std::vector<int>& cl, std::vector<int>& ist, std::vector<int>& il, std::vector<int>& fst,
+                       std::vector<int>& ff);
 
 // Restriction in device order: children of each coarse internal cell as fine
 // internal ids, ascending in fine external id (the summation order of
@@ -137,5 +223,16 @@ struct Transfer {
     std::vector<int> fineToCoarse;       // fine internal -> coarse internal
 };
 Transfer buildTransfer(const Layout& fineL, const Layout& coarseL, const std::vector<int>& restrictMap);
+
+// The same structures built on the device (setup_dev.hip), element for
+// element equal to the host builder's.  buildLayoutDevice returns false
+// (leaving the work to buildLayout) for graphs of more than 256 connected
+// components.  Matrices of at least deviceSetupMin() cells are built there
+// (UNAP_HIP_DEVICE_SETUP_MIN: 0 every matrix, negative none; default 4096).
+bool buildLayoutDevice(const HostLdu& A, int orderMode, int sigma, Layout& L);
+void pairingListsDevice(const HostLdu& A, const std::vector<double>& w, std::vector<int>& rs, std::vector<int>& nbr,
+                        std::vector<double>& wk);
+void coarseAddressingDevice(const HostLdu& F, CoarseLevel& C, const std::vector<double>& w, std::vector<double>& cw);
+int deviceSetupMin();
 
 }  // namespace unap_hip

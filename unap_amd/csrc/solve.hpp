@@ -29,6 +29,47 @@ inline int layoutMode(int order) {
     return forced >= 0 ? forced : (order == ORDER_SWEEP ? 0 : 1);
 }
 
+// A matrix's device layout: built on the device when a HIP context exists
+// and the matrix has at least deviceSetupMin() cells, on the host otherwise
+// (the two give the same layout).
+inline Layout buildLayoutAuto(const HostLdu& A, int orderMode) {
+    const int forced = forcedOrderMode();
+    if (forced >= 0) orderMode = forced;
+    const int mn = deviceSetupMin();
+    if (mn >= 0 && A.n >= mn && Ctx::peek()) {
+        StageTimer t("layout on the device (all)", A.n);
+        Layout L;
+        if (buildLayoutDevice(A, orderMode, kSortWindow, L)) return L;
+    }
+    return buildLayout(A, orderMode);
+}
+
+// galerkinListsHost's lists, built on the device into the buffers the
+// Galerkin kernels read (setup_dev.hip)
+void galerkinListsDevice(const CoarseLevel& C, const std::vector<int>& fineL, DBuf<int>& cst, DBuf<int>& cl,
+                         DBuf<int>& ist, DBuf<int>& il, DBuf<int>& fst, DBuf<int>& ff);
+// the agglomeration's device hooks: pairing lists and coarse addressing on the
+// device for levels of at least deviceSetupMin() cells when a HIP context exists
+inline AgglHooks deviceAgglHooks(bool always = false) {
+    auto on = [always](const HostLdu& A) {
+        const int mn = deviceSetupMin();
+        return always || (mn >= 0 && A.n >= mn && Ctx::peek() != nullptr);
+    };
+    AgglHooks h;
+    h.lists = [on](const HostLdu& A, const std::vector<double>& wt, std::vector<int>& rs, std::vector<int>& nbr,
+                   std::vector<double>& wk) {
+        if (!on(A)) return false;
+        pairingListsDevice(A, wt, rs, nbr, wk);
+        return true;
+    };
+    h.addressing = [on](const HostLdu& F, CoarseLevel& C, const std::vector<double>& w, std::vector<double>& cw) {
+        if (!on(F)) return false;
+        coarseAddressingDevice(F, C, w, cw);
+        return true;
+    };
+    return h;
+}
+
 struct AgglHost;
 struct LduView {
     std::unique_ptr<Layout> layout;
@@ -59,7 +100,7 @@ struct LduObj {
     LduView views[3];  // by layout mode, built on first use
     const Layout& layout(int order) {
         LduView& v = views[layoutMode(order)];
-        if (!v.layout) v.layout.reset(new Layout(buildLayout(host, layoutMode(order))));
+        if (!v.layout) v.layout.reset(new Layout(buildLayoutAuto(host, layoutMode(order))));
         return *v.layout;
     }
     DevMatrix<double>& dev64(int order) {
@@ -278,7 +319,7 @@ struct AgglHost {
             built[m].assign(levels.size(), 0);
         }
         if (!built[m][k]) {
-            layouts[m][k] = buildLayout(levels[k].A, m);
+            layouts[m][k] = buildLayoutAuto(levels[k].A, m);
             built[m][k] = 1;
         }
         return layouts[m][k];
@@ -287,8 +328,11 @@ struct AgglHost {
 inline void buildAggl(const HostLdu& fine, const std::vector<double>& w, int maxLevels, int nCoarsest,
                       AgglHost& out) {
     {
-        StageTimer t("agglomeration (host, every level)", fine.n);
-        out.levels = agglomerate(fine, w, maxLevels, nCoarsest);
+        StageTimer t("agglomeration (every level)", fine.n);
+        // pairing lists and coarse addressing on the device where a context
+        // exists (the greedy pass itself stays sequential on the host)
+        const AgglHooks hooks = deviceAgglHooks();
+        out.levels = agglomerate(fine, w, maxLevels, nCoarsest, &hooks);
     }
     for (auto& v : out.layouts) v.clear();
     for (auto& v : out.built) v.clear();

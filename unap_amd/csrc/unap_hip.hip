@@ -112,49 +112,9 @@ void AgglHost::galerkin(const HostLdu& fine, long long gen) {
         for (int k = 0; k < K; ++k) {
             const CoarseLevel& C = levels[k];
             const std::vector<int>& fl = (k == 0) ? fine.l : levels[k - 1].A.l;
-            const int nfc = (int)C.restrictMap.size(), nff = (int)C.faceRestrictMap.size();
-            const int nc = C.nCoarse, ncf = C.A.nFaces();
-            std::vector<int> cst(nc + 1, 0), cl(nfc), ist(nc + 1, 0), il, fst(ncf + 1, 0), ff;
-            for (int i = 0; i < nfc; ++i) cst[C.restrictMap[i] + 1]++;
-            for (int f = 0; f < nff; ++f) {
-                const int cf = C.faceRestrictMap[f];
-                if (cf >= 0)
-                    fst[cf + 1]++;
-                else
-                    ist[-cf]++;
-            }
-            for (int q = 0; q < nc; ++q) {
-                cst[q + 1] += cst[q];
-                ist[q + 1] += ist[q];
-            }
-            for (int q = 0; q < ncf; ++q) fst[q + 1] += fst[q];
-            il.resize(ist[nc]);
-            ff.resize(fst[ncf]);
-            std::vector<int> a(cst.begin(), cst.end() - 1), bi(ist.begin(), ist.end() - 1), bf(fst.begin(), fst.end() - 1);
-            for (int i = 0; i < nfc; ++i) cl[a[C.restrictMap[i]]++] = i;  // ascending fine cell
-            for (int f = 0; f < nff; ++f) {                               // ascending fine face
-                const int cf = C.faceRestrictMap[f];
-                if (cf < 0) {
-                    il[bi[-1 - cf]++] = f;
-                    continue;
-                }
-                // orientation (lduAgglomerateMatrix.cpp:50-66): the fine owner
-                // lies in the coarse owner, or the sides swap
-                const int ownerCell = C.restrictMap[fl[f]];
-                if (C.A.l[cf] == ownerCell)
-                    ff[bf[cf]++] = f;
-                else if (C.A.u[cf] == ownerCell)
-                    ff[bf[cf]++] = ~f;
-                else
-                    throw Error(UNAP_HIP_ERR_INTERNAL, "agglomeration: inconsistent coarse addressing");
-            }
+            StageTimer tl("Galerkin: segment lists (device)", (long long)C.restrictMap.size());
             AgglDevLevel& D = dev[k];
-            D.childStart.upload(cst, c.stream);
-            D.child.upload(cl, c.stream);
-            D.innerStart.upload(ist, c.stream);
-            D.inner.upload(il, c.stream);
-            D.cfStart.upload(fst, c.stream);
-            D.cfFine.upload(ff, c.stream);
+            galerkinListsDevice(C, fl, D.childStart, D.child, D.innerStart, D.inner, D.cfStart, D.cfFine);
             c.sync();
         }
     }
@@ -350,8 +310,9 @@ void Hierarchy<T>::setupNumeric() {
             if (bytes <= (size_t)optin &&
                 hipFuncSetAttribute((const void*)kCoarsePCGLocal, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)bytes) == hipSuccess) {
+                const Layout CLh = hostArrays(CL, c.stream);
                 std::vector<double> sv;
-                fillSellValues<double>(CL, CA, false, sv);
+                fillSellValues<double>(CLh, CA, false, sv);
                 std::vector<double> val(nnz);
                 std::vector<uint16_t> col(nnz), mid(n);
                 for (int i = 0; i < n; ++i) {
@@ -359,7 +320,7 @@ void Hierarchy<T>::setupNumeric() {
                     for (int k = 0; k < st[i + 1] - st[i]; ++k) {
                         const long long q = base + ((long long)k << kSliceShift);
                         val[st[i] + k] = sv[q];
-                        col[st[i] + k] = (uint16_t)CL.col[q];
+                        col[st[i] + k] = (uint16_t)CLh.col[q];
                     }
                     mid[i] = (uint16_t)(CL.rowInfo[i] >> 16);
                 }
@@ -1631,7 +1592,74 @@ static void profileKernel(LduObj& L, int kernel, long long reps, double* avgMs, 
     *avgMs = ms / (double)reps;
 }
 
+static int64_t layoutMismatches(const Layout& a, const Layout& b) {
+    int64_t m = 0;
+    m += a.n != b.n;
+    m += a.nf != b.nf;
+    m += a.nslices != b.nslices;
+    m += a.nnzPadded != b.nnzPadded;
+    m += a.maxRowLen != b.maxRowLen;
+    m += a.sweepable != b.sweepable;
+    m += a.perm != b.perm;
+    m += a.iperm != b.iperm;
+    m += a.sliceStart != b.sliceStart;
+    m += a.rowInfo != b.rowInfo;
+    m += a.col != b.col;
+    m += a.src != b.src;
+    m += a.fwdRows != b.fwdRows;
+    m += a.fwdStart != b.fwdStart;
+    m += a.bwdRows != b.bwdRows;
+    m += a.bwdStart != b.bwdStart;
+    return m;
+}
+
 extern "C" {
+
+int unap_hip_setup_self_check(unap_hip_ldu A, const double* weights, int64_t* mismatches) {
+    return guard([&] {
+        checkLduReady(A);
+        need(mismatches != nullptr, "null argument");
+        Ctx::get();
+        const HostLdu& H = A->o.host;
+        int64_t m = 0;
+        for (int mode = 0; mode < 3; ++mode) {
+            const Layout host = buildLayout(H, mode);
+            Layout dev;
+            if (!buildLayoutDevice(H, mode, kSortWindow, dev)) {
+                m += 1;
+                continue;
+            }
+            m += !dev.dev || !dev.col.empty() || !dev.src.empty();  // col and src stay in HBM
+            m += layoutMismatches(host, hostArrays(dev, Ctx::get().stream));
+        }
+        std::vector<double> w(H.nFaces());
+        for (int f = 0; f < H.nFaces(); ++f) w[f] = weights ? weights[f] : std::fabs(H.up[f]);
+        const AgglHooks dev = deviceAgglHooks(true);
+        const std::vector<CoarseLevel> a = agglomerate(H, w, 50, 50, nullptr);
+        const std::vector<CoarseLevel> b = agglomerate(H, w, 50, 50, &dev);
+        m += a.size() != b.size();
+        for (size_t k = 0; k < std::min(a.size(), b.size()); ++k) {
+            m += a[k].nCoarse != b[k].nCoarse;
+            m += a[k].restrictMap != b[k].restrictMap;
+            m += a[k].faceRestrictMap != b[k].faceRestrictMap;
+            m += a[k].A.l != b[k].A.l;
+            m += a[k].A.u != b[k].A.u;
+            // Galerkin segment lists
+            const std::vector<int>& fl = k == 0 ? H.l : a[k - 1].A.l;
+            std::vector<int> hl[6];
+            galerkinListsHost(a[k], fl, hl[0], hl[1], hl[2], hl[3], hl[4], hl[5]);
+            DBuf<int> dl[6];
+            galerkinListsDevice(a[k], fl, dl[0], dl[1], dl[2], dl[3], dl[4], dl[5]);
+            for (int q = 0; q < 6; ++q) {
+                std::vector<int> got(hl[q].size());
+                if (!got.empty() && dl[q].n >= got.size())
+                    UH_CHECK(hipMemcpy(got.data(), dl[q].get(), sizeof(int) * got.size(), hipMemcpyDeviceToHost));
+                m += dl[q].n < got.size() || got != hl[q];
+            }
+        }
+        *mismatches = m;
+    });
+}
 
 int unap_hip_profile_kernel(unap_hip_ldu A, int kernel, int precision, int64_t reps, double* avgMs, double* bytes) {
     return guard([&] {

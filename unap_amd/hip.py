@@ -70,6 +70,7 @@ SIGNATURES = {
     "unap_hip_ldu_read_hypre": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "unap_hip_vector_read_hypre": (ctypes.c_int, [ctypes.c_char_p, _i64, _f64p]),
     "unap_hip_rcm_order": (ctypes.c_int, [_i64, _i64, _i64p, _i64p, _i64p]),
+    "unap_hip_setup_self_check": (ctypes.c_int, [_vp, _f64p, _i64p]),
     "unap_hip_ldu_renumber": (ctypes.c_int, [_i64, _i64, _i64p, _i64p, _i64p, _f64p, _f64p, _f64p,
                                              _i64p, _i64p, _i64p, _f64p, _f64p, _f64p]),
     "unap_hip_ldu_spmv": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
@@ -280,6 +281,15 @@ def read_vector_hypre(path, n):
     out = np.empty(n)
     _check(lib().unap_hip_vector_read_hypre(os.fsencode(path), n, _p(out)))
     return out
+
+
+def setup_self_check(A, weights=None):
+    """Fields in which the device-built setup of A differs from the host-built
+    one (0: identical)."""
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    m = _i64()
+    _check(lib().unap_hip_setup_self_check(A.h, _p(w), ctypes.byref(m)))
+    return m.value
 
 
 def rcm_order(n, lower_addr, upper_addr):
