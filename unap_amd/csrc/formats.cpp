@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "host_setup.hpp"
 #include "unap_hip.h"
 
 namespace unap_hip {
@@ -105,6 +106,25 @@ Ldu cooToLdu(int64_t n, int64_t nnz, const int64_t* rows, const int64_t* cols, c
     return A;
 }
 
+// The matrix builders' conversion: on the device for matrices of at least
+// deviceSetupMin() cells where a GPU is present (setup_dev.hip, the same
+// result), on the host otherwise.
+Ldu cooToLduBuild(int64_t n, int64_t nnz, const int64_t* rows, const int64_t* cols, const double* vals,
+                  bool symmetric, int64_t base) {
+    const int mn = unap_hip::deviceSetupMin();
+    if (mn >= 0 && n >= mn && n < (1LL << 31) && nnz > 0 && rows && cols && vals) {
+        Ldu A;
+        std::string err;
+        if (unap_hip::cooToLduDevice(n, nnz, rows, cols, vals, symmetric, base, A.l, A.u, A.lo, A.d, A.up, err)) {
+            if (!err.empty()) fail(err);
+            A.n = n;
+            A.symm = symmetric;
+            return A;
+        }
+    }
+    return cooToLdu(n, nnz, rows, cols, vals, symmetric, base);
+}
+
 int create(const Ldu& A, unap_hip_ldu* out) {
     const int64_t nf = (int64_t)A.u.size();
     return unap_hip_ldu_create(A.n, nf, A.l.data(), A.u.data(), A.symm ? nullptr : A.lo.data(), A.d.data(),
@@ -164,6 +184,9 @@ int guard(F&& f) {
     } catch (const std::bad_alloc&) {
         unap_hip::setLastError("host out of memory");
         return UNAP_HIP_ERR_INTERNAL;
+    } catch (const std::exception& e) {  // the device conversion's HIP errors
+        unap_hip::setLastError(e.what());
+        return UNAP_HIP_ERR_HIP;
     }
 }
 
@@ -261,7 +284,7 @@ int unap_hip_ldu_create_coo(int64_t n, int64_t nnz, const int64_t* rows, const i
     int rc = UNAP_HIP_OK;
     const int g = guard([&] {
         if (!out) fail("out is null");
-        rc = create(cooToLdu(n, nnz, rows, cols, vals, symmetric != 0, base), out);
+        rc = create(cooToLduBuild(n, nnz, rows, cols, vals, symmetric != 0, base), out);
     });
     return g != UNAP_HIP_OK ? g : rc;
 }
@@ -281,7 +304,7 @@ int unap_hip_ldu_create_csr(int64_t n, const int64_t* row_start, const int64_t* 
             if (row_start[i + 1] < row_start[i]) fail("row starts decrease");
             for (int64_t k = row_start[i]; k < row_start[i + 1]; ++k) rows[k - row_start[0]] = i + base;
         }
-        rc = create(cooToLdu(n, nnz, rows.data(), cols, vals, symmetric != 0, base), out);
+        rc = create(cooToLduBuild(n, nnz, rows.data(), cols, vals, symmetric != 0, base), out);
     });
     return g != UNAP_HIP_OK ? g : rc;
 }
@@ -402,7 +425,7 @@ int unap_hip_ldu_read_hypre(const char* path, unap_hip_ldu* out) {
             c[k] = col - cs;
             v[k] = toScalar(tok(t, 2, "value"));
         }
-        rc = create(cooToLdu(n, nnz, r.data(), c.data(), v.data(), false, 0), out);
+        rc = create(cooToLduBuild(n, nnz, r.data(), c.data(), v.data(), false, 0), out);
     });
     return g != UNAP_HIP_OK ? g : rc;
 }

@@ -234,5 +234,10 @@ void pairingListsDevice(const HostLdu& A, const std::vector<double>& w, std::vec
                         std::vector<double>& wk);
 void coarseAddressingDevice(const HostLdu& F, CoarseLevel& C, const std::vector<double>& w, std::vector<double>& cw);
 int deviceSetupMin();
+// COO -> LDU (formats.cpp) on the device: false where no GPU is present; err
+// set when the input is refused.
+bool cooToLduDevice(int64_t n, int64_t nnz, const int64_t* rows, const int64_t* cols, const double* vals,
+                    bool symmetric, int64_t base, std::vector<int64_t>& l, std::vector<int64_t>& u,
+                    std::vector<double>& lo, std::vector<double>& d, std::vector<double>& up, std::string& err);
 
 }  // namespace unap_hip

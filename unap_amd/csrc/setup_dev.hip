@@ -529,6 +529,71 @@ __global__ void kZeroD(long long n, double* a) {
     if (i < n) a[i] = 0.0;
 }
 
+// ------------------------------------------------- COO -> LDU
+__global__ void kCooKeys(long long m, int n, const int64_t* rows, const int64_t* cols, int64_t base, uint64_t* key,
+                         long long* idx, int* bad) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int64_t r = rows[k] - base, c = cols[k] - base;
+    if (r < 0 || r >= n || c < 0 || c >= n) {
+        *bad = 1;
+        key[k] = 0;
+    } else {
+        key[k] = ((uint64_t)r << 32) | (uint64_t)c;
+    }
+    idx[k] = k;
+}
+__global__ void kCooClassify(long long m, const uint64_t* key, int* dup, int* isUpper) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > m) return;
+    if (k == m) {
+        isUpper[k] = 0;
+        return;
+    }
+    if (k > 0 && key[k] == key[k - 1]) *dup = 1;
+    isUpper[k] = (uint32_t)key[k] > (uint32_t)(key[k] >> 32);
+}
+__global__ void kCooPlace(long long m, const uint64_t* key, const long long* idx, const double* vals,
+                          const int* faceAt, int64_t* l, int64_t* u, double* up, double* d) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int r = (int)(key[k] >> 32), c = (int)(uint32_t)key[k];
+    const double v = vals[idx[k]];
+    if (c == r) {
+        d[r] = v;
+    } else if (c > r) {
+        const int f = faceAt[k];
+        l[f] = r;
+        u[f] = c;
+        up[f] = v;
+    }
+}
+// lower entry (r, c < r): its face is the upper entry (c, r), found by a
+// binary search of the sorted keys
+__global__ void kCooLower(long long m, const uint64_t* key, const long long* idx, const double* vals,
+                          const int* faceAt, double* lo, int* noUpper, int* matched) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const uint64_t kk = key[k];
+    const int r = (int)(kk >> 32), c = (int)(uint32_t)kk;
+    if (c >= r) return;
+    const uint64_t want = ((uint64_t)c << 32) | (uint64_t)r;
+    long long lo_ = 0, hi = m;
+    while (lo_ < hi) {
+        const long long mid = (lo_ + hi) >> 1;
+        if (key[mid] < want)
+            lo_ = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo_ == m || key[lo_] != want) {
+        *noUpper = 1;
+        return;
+    }
+    lo[faceAt[lo_]] = vals[idx[k]];
+    atomicAdd(matched, 1);
+}
+
 }  // namespace
 
 // Galerkin segment lists of one level on the device: children ascending by
@@ -739,6 +804,95 @@ void pairingListsDevice(const HostLdu& A, const std::vector<double>& w, std::vec
     rs = download(D.rs.get(), D.n + 1, s);
     nbr = download(dn.get(), m, s);
     wk = download(dwk.get(), m, s);
+}
+
+// cooToLdu of formats.cpp on the device: the same faces, values, checks and
+// messages.  Returns false where no GPU is present (the caller converts on
+// the host); err is set when the input is refused.
+bool cooToLduDevice(int64_t n, int64_t nnz, const int64_t* rows, const int64_t* cols, const double* vals,
+                    bool symmetric, int64_t base, std::vector<int64_t>& l, std::vector<int64_t>& u,
+                    std::vector<double>& lo, std::vector<double>& d, std::vector<double>& up, std::string& err) {
+    int devices = 0;
+    if (!Ctx::peek() && (hipGetDeviceCount(&devices) != hipSuccess || devices == 0)) {
+        (void)hipGetLastError();  // no device: the host converts
+        return false;
+    }
+    Ctx& c = Ctx::get();
+    const hipStream_t s = c.stream;
+    Temp t;
+    const long long m = nnz;
+    DBuf<int64_t> dr, dc;
+    DBuf<double> dv;
+    dr.alloc(std::max<long long>(m, 1));
+    dc.alloc(std::max<long long>(m, 1));
+    dv.alloc(std::max<long long>(m, 1));
+    dr.upload(rows, (size_t)m, s);
+    dc.upload(cols, (size_t)m, s);
+    dv.upload(vals, (size_t)m, s);
+    DBuf<uint64_t> k0(std::max<long long>(m, 1)), k1(std::max<long long>(m, 1));
+    DBuf<long long> i0(std::max<long long>(m, 1)), i1(std::max<long long>(m, 1));
+    DBuf<int> flags(4);  // out of range, duplicate, lower without upper, matched lowers
+    UH_CHECK(hipMemsetAsync(flags.get(), 0, sizeof(int) * 4, s));
+    if (m > 0) {
+        kCooKeys<<<blocksFor(m), kT, 0, s>>>(m, (int)n, dr.get(), dc.get(), base, k0.get(), i0.get(), flags.get());
+        UH_LAUNCHED();
+    }
+    if (readOne(flags.get(), s)) {
+        err = "COO index out of range";
+        return true;
+    }
+    if (m > 0) {
+        size_t bytes = 0;
+        const unsigned endBit = 32 + bitsFor(n);
+        UH_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, k0.get(), k1.get(), i0.get(), i1.get(), (size_t)m, 0u, endBit,
+                                           s));
+        UH_CHECK(rocprim::radix_sort_pairs(t.get(bytes), bytes, k0.get(), k1.get(), i0.get(), i1.get(), (size_t)m, 0u,
+                                           endBit, s));
+    }
+    DBuf<int> isUpper(m + 1), faceAt(m + 1);
+    kCooClassify<<<blocksFor(m + 1), kT, 0, s>>>(m, k1.get(), flags.get() + 1, isUpper.get());
+    UH_LAUNCHED();
+    if (readOne(flags.get() + 1, s)) {
+        err = "duplicate COO entry";
+        return true;
+    }
+    exclusiveSum(t, isUpper.get(), faceAt.get(), m + 1, s);
+    const int nf = readOne(faceAt.get() + m, s);
+    DBuf<int64_t> dl(std::max(nf, 1)), du(std::max(nf, 1));
+    DBuf<double> dup(std::max(nf, 1)), dlo(std::max(nf, 1)), dd(std::max<long long>(n, 1));
+    kZeroD<<<blocksFor(n), kT, 0, s>>>(n, dd.get());
+    UH_LAUNCHED();
+    if (m > 0) {
+        kCooPlace<<<blocksFor(m), kT, 0, s>>>(m, k1.get(), i1.get(), dv.get(), faceAt.get(), dl.get(), du.get(),
+                                              dup.get(), dd.get());
+        UH_LAUNCHED();
+    }
+    if (!symmetric) {
+        kZeroD<<<blocksFor(nf), kT, 0, s>>>(nf, dlo.get());
+        UH_LAUNCHED();
+        if (m > 0) {
+            kCooLower<<<blocksFor(m), kT, 0, s>>>(m, k1.get(), i1.get(), dv.get(), faceAt.get(), dlo.get(),
+                                                  flags.get() + 2, flags.get() + 3);
+            UH_LAUNCHED();
+        }
+        const std::vector<int> f = download(flags.get(), 4, s);
+        if (f[2]) {
+            err = "COO structure is not symmetric (a lower entry has no upper one)";
+            return true;
+        }
+        if (f[3] != nf) {
+            err = "COO structure is not symmetric (an upper entry has no lower one)";
+            return true;
+        }
+        lo = download(dlo.get(), nf, s);
+    } else {
+        lo.clear();
+    }
+    l = download(dl.get(), nf, s);
+    u = download(du.get(), nf, s);
+    up = download(dup.get(), nf, s);
+    d = download(dd.get(), n, s);
+    return true;
 }
 
 // Smallest matrix built on the device (UNAP_HIP_DEVICE_SETUP_MIN: cells;
