@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--mixed-n-coarsest", type=int, default=2000,
                     help="coarsest-level threshold of the mixed algorithm (dense FP64 inverse)")
     ap.add_argument("--pair-steps", type=int, default=2, help="pairwise steps per level of the mixed hierarchy")
+    ap.add_argument("--mixed-scaling", type=int, default=0,
+                    help="correction scaling of the mixed cycle: 0 none (FCG's step length already scales each "
+                         "correction), 1 coarse levels, 2 every level but the two coarsest (the reference)")
     ap.add_argument("--fp64-steps", type=int, default=3,
                     help="timed solves of each FP64 leg (median; one untimed warm-up solve before them)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
@@ -74,6 +77,8 @@ def main():
     ap.add_argument("--reps", type=int, default=5, help="launches of --kernel-only")
     ap.add_argument("--mixed-only", action="store_true",
                     help="stop after the timed mixed solves (for kernel traces of exactly those solves)")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="time only the FP64 drop-in solve (for kernel traces of exactly that solve)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
@@ -109,7 +114,7 @@ def main():
 
     common = dict(smoother=2, n_vcycles=1, n_post_sweeps=4, n_finest_sweeps=4)
     mixed = dict(level_sweep_growth=0, pair_steps=args.pair_steps, n_cells_coarsest=args.mixed_n_coarsest,
-                 cycle=hip.CYCLE_FIXED, **common)
+                 cycle=hip.CYCLE_FIXED, scaling=args.mixed_scaling, **common)
     t0 = time.time()
     P32 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, **mixed))
     t_setup32 = time.time() - t0
@@ -120,7 +125,7 @@ def main():
     t_setup64 = time.time() - t0
     log(f"fp64 hierarchy setup {t_setup64:.1f}s")
     P64f = None
-    if not args.mixed_only:
+    if not (args.mixed_only or args.fp64_only):
         P64f = hip.Precond(A, "AMG", amg=hip.amg_params(precision=64, **mixed))
 
     b = hip.DeviceArray.from_numpy(s.b)
@@ -138,6 +143,16 @@ def main():
         return time.perf_counter() - t, perf
 
     levels32 = P32.amg_levels()
+    if args.fp64_only:
+        solve(S64)  # warm-up
+        times64 = []
+        for k in range(max(1, args.fp64_steps)):
+            dt, p64 = solve(S64)
+            times64.append(dt)
+            log(f"fp64 step {k}: {dt * 1e3:.1f} ms, {p64['nIterations']} iterations, converged={p64['converged']}")
+        print(json.dumps({"fp64_s": times64, "iterations": p64["nIterations"], "hierarchy": P64.amg_levels(),
+                          "n_post_sweeps": 4, "level_sweep_growth": 1, "n_finest_sweeps": 4}), flush=True)
+        return
     for _ in range(args.warmup):
         solve(S32)
     log("warmup done")
@@ -226,7 +241,8 @@ def main():
         "config": {"workload": f"C4: {N}^3 permuted log-uniform(1e-4..1) Poisson LDU, AMG-PCG to rel.res 1e-8",
                    "n_cells": s.n, "n_faces": s.nfaces,
                    "mixed": f"FCG(FP64) + FP32 AMG V(0,4) Chebyshev(4), {args.pair_steps} pairwise steps a level, "
-                            f"coarsest threshold {args.mixed_n_coarsest}, dense FP64 inverse (GPU Gauss-Jordan)",
+                            f"coarsest threshold {args.mixed_n_coarsest}, dense FP64 inverse (GPU Gauss-Jordan), "
+                            f"correction scaling {['none', 'coarse levels', 'reference'][args.mixed_scaling]}",
                    "fp64": "PCG + FP64 AMG V(0,4+level) Chebyshev(4), reference pairwise hierarchy, coarsest PCG+DIC 1e-2",
                    "fp64_same_algorithm": "the mixed algorithm with the hierarchy in FP64",
                    "n_cells_coarsest_fp64": args.n_coarsest, "n_cells_coarsest_mixed": args.mixed_n_coarsest},

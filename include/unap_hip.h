@@ -155,6 +155,10 @@ typedef struct {
                                        PCG/PBiCGStab to 1e-2) or a fixed one (estimates once at
                                        setup, coarsest level by a dense FP64 inverse); default
                                        UNAP_HIP_CYCLE_AUTO: reference in FP64, fixed in FP32 */
+    int64_t scaling;                /* correction scaling (scalingFactor.cpp, symmetric matrices):
+                                       2 every level but the two coarsest (the reference,
+                                       Vcycle.cpp:72,193; default), 1 the coarse levels only (not
+                                       the finest), 0 none */
 } unap_hip_amg_params;
 #define UNAP_HIP_CYCLE_AUTO (-1)
 #define UNAP_HIP_CYCLE_REFERENCE 0

@@ -306,6 +306,28 @@ def test_mixed_fcg_accuracy(oracle, precision, pair_steps):
     assert lv[0] == (s.n, s.nfaces) and all(a[0] > b[0] for a, b in zip(lv, lv[1:]))
 
 
+@pytest.mark.parametrize("scaling", [0, 1])
+def test_mixed_fcg_scaling(oracle, scaling):
+    # the mixed cycle without correction scaling (0) or with it on the coarse
+    # levels only (1): the same accuracy bar as the reference scaling; the
+    # unscaled cycle ends with the prolongation written straight into psi
+    s = _sys(4, 20)
+    A = hip.LduMatrix.from_system(s)
+    P = hip.Precond(A, "AMG", amg=hip.amg_params(smoother=2, n_cells_coarsest=50, n_vcycles=1, n_post_sweeps=4,
+                                                   n_finest_sweeps=4, precision=32, pair_steps=2,
+                                                   cycle=hip.CYCLE_FIXED, scaling=scaling))
+    x, p = hip.Solver("FCG", P, rel_tol=1e-8, max_iter=300).solve(A, s.b)
+    assert p["converged"]
+    M = oracle.matrix(s)
+    xo, _ = M.solve("PCG", "DIC", s.b, rel_tol=1e-13, max_iter=5000)
+    x8, _ = M.solve("PCG", "DIC", s.b, rel_tol=1e-8, max_iter=5000)
+    assert np.linalg.norm(s.b - M.spmv(x)) <= 2e-8 * np.linalg.norm(s.b)
+    e_mixed = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    assert e_mixed <= 10 * np.linalg.norm(x8 - xo) / np.linalg.norm(xo) + 1e-12
+    with pytest.raises(hip.UnapHipError):
+        hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, cycle=hip.CYCLE_FIXED, scaling=3))
+
+
 @pytest.mark.parametrize("n", [1, 31, 64, 100, 700])
 def test_dense_inverse_spd(n):
     # blocked Gauss-Jordan with FP64 MFMA updates against numpy's inverse

@@ -108,6 +108,20 @@ void launchRows(const Body& b, int n, Skip sk = {nullptr, nullptr}) {
     kRows<Body><<<L.grid, kBlock, 0, c.stream>>>(b, n, sk);
     UH_LAUNCHED();
 }
+// launchRows with the rows in XCD slabs (kRowsXcd), for gathers whose cache
+// lines neighbouring rows share; small row counts take the plain walk.
+template <class Body>
+void launchRowsXcd(const Body& b, int n, Skip sk = {nullptr, nullptr}) {
+    if (n <= 0) return;
+    const int g = rowLaunch(n).grid;
+    if (g < 64) {
+        launchRows(b, n, sk);
+        return;
+    }
+    Ctx& c = Ctx::get();
+    kRowsXcd<Body><<<(g + 7) / 8 * 8, kBlock, 0, c.stream>>>(b, n, sk);
+    UH_LAUNCHED();
+}
 
 // Row kernel with NS sums followed by the one-block finalize.  `order` is the
 // level's external->internal map, walked by the reference-order mode.

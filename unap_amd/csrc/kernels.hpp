@@ -245,6 +245,21 @@ __global__ void __launch_bounds__(kBlock) kRows(Body body, int n, Skip skip) {
     UH_ROWS(r, n) body(r);
 }
 
+// Rows in XCD slabs.  Workgroups go to the eight XCDs in turn (workgroup b
+// to XCD b mod 8), so the grid-stride walk above spreads consecutive rows
+// over all eight L2 caches.  Here, in each step of the walk, XCD x takes the
+// x-th eighth of the rows in flight as one contiguous slab, and its L2
+// caches only the gathers of that slab.  The grid is a multiple of 8; every
+// row is still visited exactly once.
+template <class Body>
+__global__ void __launch_bounds__(kBlock) kRowsXcd(Body body, int n, Skip skip) {
+    if (skip.on()) return;
+    const int per = gridDim.x >> 3;
+    const long long lane = ((long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3)) * kBlock + threadIdx.x;
+    const long long step = (long long)gridDim.x * kBlock;
+    for (long long r = lane; r < n; r += step) body((int)r);
+}
+
 // Reference-order reduction: body.terms(i) summed over the caller's cell
 // order (internal id iperm[e] for e = 0, 1, ...), exactly the serial
 // left-to-right loops of the reference (unapVectorOperations.cpp:21-30).

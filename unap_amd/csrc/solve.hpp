@@ -348,6 +348,7 @@ struct AmgParams {
     int maxLevels = 50, nCoarsest = 50;
     int pairSteps = 1;  // agglomeration steps per hierarchy level
     bool fixedCycle = false;  // estimates at setup, dense FP64 coarsest solve
+    int scaling = 2;          // 2 all but the two coarsest levels, 1 coarse levels only, 0 none
 };
 
 // The V-cycle of MGSolver (Vcycle.cpp:27-332) in precision T on a resident
@@ -366,7 +367,8 @@ struct Hierarchy {
     std::vector<DBuf<T>> pre;                         // pre-smoothed corrections (nPre > 0)
     AmgParams prm;
     int order = ORDER_STREAM;  // GS hierarchies sweep: ORDER_SWEEP
-    bool scale = true;
+    bool scale = true;        // coarse-level correction scaling
+    bool scaleFinest = true;  // and on the finest level
     int L = 0;
     // smoothers: index 0 finest, l+1 coarse level l
     std::vector<char> firstCall;

@@ -204,7 +204,8 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     L = (int)sel.size();
     if (L == 0)
         throw Error(-4, "agglomeration produced no coarse level (matrix smaller than the coarsest-level size)");
-    scale = A.host.symm;  // scaleCorrection_(A.symm()), MG.cpp:14
+    scale = A.host.symm && p.scaling >= 1;  // scaleCorrection_(A.symm()), MG.cpp:14
+    scaleFinest = A.host.symm && p.scaling >= 2;
     const bool fp32 = sizeof(T) == 4;
     const bool fixed = prm.fixedCycle;
     if (fp32 && !fixed) throw Error(UNAP_HIP_ERR_ARG, "the FP32 hierarchy needs the fixed cycle");
@@ -446,7 +447,7 @@ template <typename T>
 void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, bool psiZero, Skip sk) {
     const int coarsest = L - 1;
     // restrict the finest residual (Vcycle.cpp:43-45)
-    launchRows(BRestrict<T, T>{cs[0].get(), ch[0].get(), fRes, work[0].b.get(), nullptr}, nl(0), sk);
+    launchRowsXcd(BRestrict<T, T>{cs[0].get(), ch[0].get(), fRes, work[0].b.get(), nullptr}, nl(0), sk);
     for (int l = 0; l < coarsest; ++l) {
         LevelWork<T>& W = work[l];
         const int n = nl(l);
@@ -461,7 +462,7 @@ void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, bool psiZero, Skip s
             }
             launchRows(BSubAmul<T>{mat(l).view(), W.a.get(), W.b.get()}, n, sk);
         }
-        launchRows(BRestrict<T, T>{cs[l + 1].get(), ch[l + 1].get(), W.b.get(), work[l + 1].b.get(), nullptr},
+        launchRowsXcd(BRestrict<T, T>{cs[l + 1].get(), ch[l + 1].get(), W.b.get(), work[l + 1].b.get(), nullptr},
                    nl(l + 1), sk);
     }
     coarsestSolve(sk);
@@ -469,7 +470,7 @@ void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, bool psiZero, Skip s
         LevelWork<T>& W = work[l];
         const int n = nl(l);
         if (prm.nPre) launchRows(BCopy<T, T>{W.a.get(), pre[l].get()}, n, sk);
-        launchRows(BProlong<T>{f2c[l + 1].get(), work[l + 1].a.get(), W.a.get()}, n, sk);
+        launchRowsXcd(BProlong<T>{f2c[l + 1].get(), work[l + 1].a.get(), W.a.get()}, n, sk);
         if (scale && l < coarsest - 1) {
             sfAmul(mat(l), W.a.get(), W.b.get(), W.c.get(), FinSF{mgst.get(), l}, sk);
             launchRows(BJacobiScale<T>{W.a.get(), W.b.get(), W.c.get(), mat(l).diag.get(), mgst.get(), l,
@@ -481,15 +482,18 @@ void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, bool psiZero, Skip s
         smooth(l + 1, W.a.get(), W.b.get(), prm.nPost + prm.growth * l, sk);
     }
     const int n = fine->n;
-    launchRows(BProlong<T>{f2c[0].get(), work[0].a.get(), fw.a.get()}, n, sk);
-    if (scale) {
-        sfAmul(*fine, fw.a.get(), fRes, fw.b.get(), FinSF{mgst.get(), 63}, sk);
-        launchRows(BPsiScaled<T>{psi, fw.a.get(), fRes, fw.b.get(), fine->diag.get(), mgst.get(), 63, psiZero}, n,
-                   sk);
-    } else if (psiZero) {
-        launchRows(BCopy<T, T>{fw.a.get(), psi}, n, sk);
+    if (!scaleFinest && psiZero) {
+        // unscaled correction onto a zero psi: prolong straight into it
+        launchRowsXcd(BProlong<T>{f2c[0].get(), work[0].a.get(), psi}, n, sk);
     } else {
-        launchRows(BAddTo<T>{psi, fw.a.get()}, n, sk);
+        launchRowsXcd(BProlong<T>{f2c[0].get(), work[0].a.get(), fw.a.get()}, n, sk);
+        if (scaleFinest) {
+            sfAmul(*fine, fw.a.get(), fRes, fw.b.get(), FinSF{mgst.get(), 63}, sk);
+            launchRows(BPsiScaled<T>{psi, fw.a.get(), fRes, fw.b.get(), fine->diag.get(), mgst.get(), 63, psiZero},
+                       n, sk);
+        } else {
+            launchRows(BAddTo<T>{psi, fw.a.get()}, n, sk);
+        }
     }
     smooth(0, psi, source, prm.nFinest, sk);
 }
@@ -1120,6 +1124,7 @@ int unap_hip_amg_params_default(unap_hip_amg_params* p) {
         p->level_sweep_growth = 1;
         p->pair_steps = 1;
         p->cycle = UNAP_HIP_CYCLE_AUTO;
+        p->scaling = 2;
     });
 }
 
@@ -1136,6 +1141,8 @@ static AmgParams toParams(const unap_hip_amg_params* p) {
     a.precision = (int)p->precision;
     a.growth = (int)p->level_sweep_growth;
     a.pairSteps = (int)p->pair_steps;
+    a.scaling = (int)p->scaling;
+    need(a.scaling >= 0 && a.scaling <= 2, "scaling must be 0, 1 or 2");
     need(a.growth == 0 || a.growth == 1, "level_sweep_growth must be 0 or 1");
     need(a.pairSteps >= 1 && a.pairSteps <= 3, "pair_steps must be 1, 2 or 3");
     need(p->cycle >= -1 && p->cycle <= 1, "unknown cycle");

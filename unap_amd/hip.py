@@ -32,7 +32,7 @@ class AmgParams(ctypes.Structure):
     _fields_ = [("smoother", _i64), ("n_cells_coarsest", _i64), ("max_levels", _i64),
                 ("n_vcycles", _i64), ("n_pre_sweeps", _i64), ("n_post_sweeps", _i64),
                 ("n_finest_sweeps", _i64), ("precision", _i64), ("level_sweep_growth", _i64),
-                ("pair_steps", _i64), ("cycle", _i64)]
+                ("pair_steps", _i64), ("cycle", _i64), ("scaling", _i64)]
 
 
 class Perf(ctypes.Structure):
