@@ -77,9 +77,7 @@ def solve_start(rows):
     its initial-residual finalizer (eigen estimates inside a solve launch
     kStateInit too)."""
     fin = [i for i, r in enumerate(rows) if "FinInitRes" in r["Kernel_Name"]]
-    if not fin:
-        raise SystemExit("no initial-residual dispatch in the trace")
-    i = fin[-1]
+    i = fin[-1] if fin else len(rows) - 1  # without it: the last kStateInit
     while i >= 0 and "kStateInit" not in rows[i]["Kernel_Name"]:
         i -= 1
     if i < 0:
