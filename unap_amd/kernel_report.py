@@ -87,8 +87,9 @@ def solve_start(rows):
 
 def normalize(name):
     """One name for the load-shape instantiations of a kernel body
-    (BCheby<float, 0> and BCheby<float, 1> -> BCheby<float>)."""
-    return re.sub(r"<(float|double), \d+>", r"<\1>", name)
+    (BCheby<float, 0> and BCheby<float, 1> -> BCheby<float>) and for its two
+    row walks (kRowsXcd, the XCD-slab walk of the large levels -> kRows)."""
+    return re.sub(r"<(float|double), \d+>", r"<\1>", name).replace("kRowsXcd<", "kRows<")
 
 
 def short(name):
