@@ -1540,13 +1540,17 @@ static void profileKernel(LduObj& L, int kernel, long long reps, double* avgMs, 
         M64.buildValU(L.host, lay, c.stream);
     }
     const Skip none{nullptr, nullptr};
+    long long launch = 0;
     auto once = [&]() {
+        const bool odd = (launch++ & 1) != 0;
         switch (kernel) {
             case UNAP_HIP_KERNEL_AMUL:
                 launchRows(BAmul<T>{M.view(), x.get(), y.get()}, n);
                 break;
-            case UNAP_HIP_KERNEL_CHEBY:
-                launchRows(BCheby<T>{M.view(), x.get(), y.get(), b.get(), w.get(), est, 1}, n);
+            case UNAP_HIP_KERNEL_CHEBY:  // as the large levels' sweeps run in a solve: x ping-ponged
+                launchRowsXcd(BCheby<T>{M.view(), odd ? y.get() : x.get(), odd ? x.get() : y.get(), b.get(), w.get(),
+                                        est, 1},
+                              n);
                 break;
             case UNAP_HIP_KERNEL_GS:
                 gsSmooth<T>(M, x.get(), b.get(), 1, none);
