@@ -8,9 +8,9 @@ residual 1e-8 from x = 0.
   * mixed (the headline `value`): flexible CG in FP64 (true residual,
     dot products and norms in FP64) around one V-cycle of the agglomeration
     AMG in FP32 (every second level of the reference's pairwise hierarchy,
-    Chebyshev(4) smoothers on every level with estimates taken at setup,
-    restriction and prolongation in FP32; coarsest level solved in FP64 by a
-    dense inverse);
+    Chebyshev(4) smoothers on every level with estimates taken at setup over
+    the interval [lambda / 5, lambda], restriction and prolongation in FP32;
+    coarsest level solved in FP64 by a dense inverse);
   * fp64 drop-in: the reference's algorithm in FP64 on the GPU (PCG around
     MGSolver V-cycles over the full pairwise hierarchy with Chebyshev
     smoothers, nPost = nFinest = 4 with the reference's nPost + level
@@ -50,6 +50,10 @@ def main():
     ap.add_argument("--mixed-n-coarsest", type=int, default=2000,
                     help="coarsest-level threshold of the mixed algorithm (dense FP64 inverse)")
     ap.add_argument("--pair-steps", type=int, default=2, help="pairwise steps per level of the mixed hierarchy")
+    ap.add_argument("--mixed-cheby-ratio", type=float, default=5.0,
+                    help="Chebyshev interval of the mixed cycle: lower end lambda / ratio (the reference: 1.5)")
+    ap.add_argument("--mixed-cheby-boost", type=float, default=1.0,
+                    help="Chebyshev interval of the mixed cycle: upper end lambda * boost (the reference: 1.1)")
     ap.add_argument("--mixed-scaling", type=int, default=0,
                     help="correction scaling of the mixed cycle: 0 none (FCG's step length already scales each "
                          "correction), 1 coarse levels, 2 every level but the two coarsest (the reference)")
@@ -114,7 +118,8 @@ def main():
 
     common = dict(smoother=2, n_vcycles=1, n_post_sweeps=4, n_finest_sweeps=4)
     mixed = dict(level_sweep_growth=0, pair_steps=args.pair_steps, n_cells_coarsest=args.mixed_n_coarsest,
-                 cycle=hip.CYCLE_FIXED, scaling=args.mixed_scaling, **common)
+                 cycle=hip.CYCLE_FIXED, scaling=args.mixed_scaling, cheby_eig_ratio=args.mixed_cheby_ratio,
+                 cheby_boost=args.mixed_cheby_boost, **common)
     t0 = time.time()
     P32 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, **mixed))
     t_setup32 = time.time() - t0
@@ -240,10 +245,12 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"C4: {N}^3 permuted log-uniform(1e-4..1) Poisson LDU, AMG-PCG to rel.res 1e-8",
                    "n_cells": s.n, "n_faces": s.nfaces,
-                   "mixed": f"FCG(FP64) + FP32 AMG V(0,4) Chebyshev(4), {args.pair_steps} pairwise steps a level, "
+                   "mixed": f"FCG(FP64) + FP32 AMG V(0,4) Chebyshev(4) over [lambda/{args.mixed_cheby_ratio:g}, "
+                            f"{args.mixed_cheby_boost:g} lambda], {args.pair_steps} pairwise steps a level, "
                             f"coarsest threshold {args.mixed_n_coarsest}, dense FP64 inverse (GPU Gauss-Jordan), "
                             f"correction scaling {['none', 'coarse levels', 'reference'][args.mixed_scaling]}",
-                   "fp64": "PCG + FP64 AMG V(0,4+level) Chebyshev(4), reference pairwise hierarchy, coarsest PCG+DIC 1e-2",
+                   "fp64": "PCG + FP64 AMG V(0,4+level) Chebyshev(4) over [lambda/1.5, 1.1 lambda] (the reference's), "
+                           "reference pairwise hierarchy, coarsest PCG+DIC 1e-2",
                    "fp64_same_algorithm": "the mixed algorithm with the hierarchy in FP64",
                    "n_cells_coarsest_fp64": args.n_coarsest, "n_cells_coarsest_mixed": args.mixed_n_coarsest},
         "hierarchy_levels": levels32,

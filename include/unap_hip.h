@@ -159,6 +159,13 @@ typedef struct {
                                        2 every level but the two coarsest (the reference,
                                        Vcycle.cpp:72,193; default), 1 the coarse levels only (not
                                        the finest), 0 none */
+    double cheby_eig_ratio;         /* Chebyshev smoothing interval of every level:
+                                       [lambda / cheby_eig_ratio, lambda * cheby_boost], lambda the
+                                       level's eigenDiagPCG estimate.  Defaults 1.5 and 1.1, the
+                                       reference's eigRatioCheby_ and boostFactorCheby_
+                                       (chebySmoother.hpp:46-47, chebySmoother.cpp:63-64);
+                                       cheby_eig_ratio > 1, cheby_boost > 0 */
+    double cheby_boost;
 } unap_hip_amg_params;
 #define UNAP_HIP_CYCLE_AUTO (-1)
 #define UNAP_HIP_CYCLE_REFERENCE 0

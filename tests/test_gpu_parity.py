@@ -328,6 +328,30 @@ def test_mixed_fcg_scaling(oracle, scaling):
         hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, cycle=hip.CYCLE_FIXED, scaling=3))
 
 
+def test_mixed_fcg_cheby_interval(oracle):
+    # the mixed cycle's Chebyshev smoothers over [lambda/5, lambda] instead of
+    # the reference's [lambda/1.5, 1.1 lambda]: the same accuracy bar, and on
+    # this C4 system no more FCG iterations than the reference interval
+    s = _sys(4, 24)
+    A = hip.LduMatrix.from_system(s)
+    base = dict(smoother=2, n_cells_coarsest=50, n_vcycles=1, n_post_sweeps=4, n_finest_sweeps=4, precision=32,
+                pair_steps=2, cycle=hip.CYCLE_FIXED, scaling=0)
+    its = {}
+    for ratio, boost in ((1.5, 1.1), (5.0, 1.0)):
+        P = hip.Precond(A, "AMG", amg=hip.amg_params(cheby_eig_ratio=ratio, cheby_boost=boost, **base))
+        x, p = hip.Solver("FCG", P, rel_tol=1e-8, max_iter=300).solve(A, s.b)
+        assert p["converged"]
+        its[ratio] = p["nIterations"]
+        M = oracle.matrix(s)
+        xo, _ = M.solve("PCG", "DIC", s.b, rel_tol=1e-13, max_iter=5000)
+        x8, _ = M.solve("PCG", "DIC", s.b, rel_tol=1e-8, max_iter=5000)
+        assert np.linalg.norm(s.b - M.spmv(x)) <= 2e-8 * np.linalg.norm(s.b)
+        e_mixed = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        assert e_mixed <= 10 * np.linalg.norm(x8 - xo) / np.linalg.norm(xo) + 1e-12
+    print("FCG iterations by Chebyshev interval:", its)
+    assert its[5.0] <= its[1.5]
+
+
 @pytest.mark.parametrize("n", [1, 31, 64, 100, 700])
 def test_dense_inverse_spd(n):
     # blocked Gauss-Jordan with FP64 MFMA updates against numpy's inverse

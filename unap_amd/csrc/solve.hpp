@@ -349,6 +349,9 @@ struct AmgParams {
     int pairSteps = 1;  // agglomeration steps per hierarchy level
     bool fixedCycle = false;  // estimates at setup, dense FP64 coarsest solve
     int scaling = 2;          // 2 all but the two coarsest levels, 1 coarse levels only, 0 none
+    // Chebyshev interval [lambda / chebyRatio, lambda * chebyBoost] of every
+    // level (chebySmoother.cpp:63-64; the reference's 1.5 and 1.1 by default)
+    double chebyRatio = 1.5, chebyBoost = 1.1;
 };
 
 // The V-cycle of MGSolver (Vcycle.cpp:27-332) in precision T on a resident

@@ -373,7 +373,7 @@ void Hierarchy<T>::setupNumeric() {
                 bb.upload(bh, c.stream);
                 launchRows(BFill<T>{W.a.get(), (T)0}, M.n);
                 eigenEstimate<T>(M, W.a.get(), bb.get(), 10, est.get() + si, W.b.get(), W.c.get(),
-                                 W.d.get(), Skip{nullptr, nullptr});
+                                 W.d.get(), Skip{nullptr, nullptr}, prm.chebyBoost, prm.chebyRatio);
                 c.sync();
             }
             for (auto& f : firstCall) f = 0;
@@ -394,7 +394,7 @@ void Hierarchy<T>::smooth(int si, T* x, const T* b, int nSweeps, Skip sk) {
     T* v2 = (si == 0) ? W.b.get() : W.d.get();
     T* v3 = (si == 0) ? W.c.get() : W.e.get();
     if (firstCall[si] && !prm.fixedCycle) {
-        eigenEstimate<T>(M, x, b, 10, est.get() + si, v1, v2, v3, sk);
+        eigenEstimate<T>(M, x, b, 10, est.get() + si, v1, v2, v3, sk, prm.chebyBoost, prm.chebyRatio);
         firstCall[si] = 0;
         return;
     }
@@ -1125,6 +1125,8 @@ int unap_hip_amg_params_default(unap_hip_amg_params* p) {
         p->pair_steps = 1;
         p->cycle = UNAP_HIP_CYCLE_AUTO;
         p->scaling = 2;
+        p->cheby_eig_ratio = 1.5;
+        p->cheby_boost = 1.1;
     });
 }
 
@@ -1142,7 +1144,11 @@ static AmgParams toParams(const unap_hip_amg_params* p) {
     a.growth = (int)p->level_sweep_growth;
     a.pairSteps = (int)p->pair_steps;
     a.scaling = (int)p->scaling;
+    a.chebyRatio = p->cheby_eig_ratio;
+    a.chebyBoost = p->cheby_boost;
     need(a.scaling >= 0 && a.scaling <= 2, "scaling must be 0, 1 or 2");
+    need(std::isfinite(a.chebyRatio) && a.chebyRatio > 1.0 && std::isfinite(a.chebyBoost) && a.chebyBoost > 0.0,
+         "cheby_eig_ratio must exceed 1 and cheby_boost be positive");
     need(a.growth == 0 || a.growth == 1, "level_sweep_growth must be 0 or 1");
     need(a.pairSteps >= 1 && a.pairSteps <= 3, "pair_steps must be 1, 2 or 3");
     need(p->cycle >= -1 && p->cycle <= 1, "unknown cycle");
@@ -1161,6 +1167,8 @@ int unap_hip_precond_create(unap_hip_ldu A, int type, const unap_hip_amg_params*
         checkLduReady(A);
         need(out != nullptr, "out is null");
         need(type >= 0 && type <= 4, "unknown preconditioner type");
+        // the parameters are checked before anything touches the device
+        const AmgParams prm = (type == UNAP_HIP_PRECOND_AMG) ? toParams(amg) : AmgParams{};
         std::unique_ptr<unap_hip_precond_s> P(new unap_hip_precond_s);
         PrecondObj& o = P->o;
         o.type = type;
@@ -1201,7 +1209,6 @@ int unap_hip_precond_create(unap_hip_ldu A, int type, const unap_hip_amg_params*
             }
             c.sync();
         } else if (type == UNAP_HIP_PRECOND_AMG) {
-            AmgParams prm = toParams(amg);
             o.order = (prm.smoother == UNAP_HIP_SMOOTHER_GS) ? ORDER_SWEEP : ORDER_STREAM;
             auto ag = agglFor(A->o, fw, false, prm.maxLevels, prm.nCoarsest);
             if (prm.precision == 32) {
