@@ -147,9 +147,11 @@ typedef struct {
     int64_t precision;              /* 64 or 32 */
     int64_t level_sweep_growth;     /* 1: coarse level l smooths nPost + l times (Vcycle.cpp:250,
                                        default); 0: every coarse level smooths nPost times */
-    int64_t pair_steps;             /* pairwise agglomeration steps per hierarchy level: 1 (the
-                                       reference, default) or 2 (keep every second level of the
-                                       reference hierarchy, restriction maps composed) */
+    int64_t pair_steps;             /* pairwise agglomeration steps per hierarchy level, 1 (the
+                                       reference, default) to 8: the hierarchy keeps levels
+                                       pair_steps, 2 pair_steps, ... of the reference's pairwise
+                                       agglomeration (counted from the finest) and its coarsest
+                                       level, restriction maps composed */
     int64_t cycle;                  /* UNAP_HIP_CYCLE_*: the reference V-cycle (Chebyshev
                                        estimates on each solve's first call, coarsest level by
                                        PCG/PBiCGStab to 1e-2) or a fixed one (estimates once at

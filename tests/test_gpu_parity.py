@@ -328,6 +328,27 @@ def test_mixed_fcg_scaling(oracle, scaling):
         hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, cycle=hip.CYCLE_FIXED, scaling=3))
 
 
+@pytest.mark.parametrize("cfg,N", [(4, 24), (2, 20)])
+def test_pair_steps_levels(cfg, N):
+    # pair_steps k keeps agglomeration levels k, 2k, ... (from the finest)
+    # and the coarsest one, whatever the agglomeration's depth
+    s = _sys(cfg, N)
+    A = hip.LduMatrix.from_system(s)
+    G = hip.Agglomeration(A, np.abs(s.upper), n_coarsest=50)
+    sizes, n, nf = [], s.n, s.nfaces
+    for k in range(G.nlevels()):
+        L = G.level(k, n, nf, values=False)
+        n, nf = L["n"], L["nfaces"]
+        sizes.append((n, nf))
+    nl = len(sizes)
+    assert nl >= 5
+    for ps in (1, 2, 3, 4):
+        P = hip.Precond(A, "AMG", amg=hip.amg_params(smoother=2, n_cells_coarsest=50, n_vcycles=1, precision=32,
+                                                       pair_steps=ps, cycle=hip.CYCLE_FIXED))
+        want = [(s.n, s.nfaces)] + [sizes[k] for k in list(range(ps - 1, nl - 1, ps)) + [nl - 1]]
+        assert [tuple(v) for v in P.amg_levels()] == want, ps
+
+
 def test_mixed_fcg_cheby_interval(oracle):
     # the mixed cycle's Chebyshev smoothers over [lambda/5, lambda] instead of
     # the reference's [lambda/1.5, 1.1 lambda]: the same accuracy bar, and on

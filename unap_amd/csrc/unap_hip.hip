@@ -196,11 +196,15 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     fine = &Fine<T>::get(A, order);
     ag = agg;
     prm = p;
-    // hierarchy levels: every pairSteps-th agglomeration level, ending at the
-    // coarsest one
+    // hierarchy levels: agglomeration levels pairSteps, 2 pairSteps, ...
+    // (counted from the finest) and the coarsest one, so the first coarse
+    // level is always pairSteps steps down whatever the agglomeration's depth
     sel.clear();
-    for (int k = (int)ag->levels.size() - 1; k >= 0; k -= p.pairSteps) sel.push_back(k);
-    std::reverse(sel.begin(), sel.end());
+    {
+        const int nl = (int)ag->levels.size();
+        for (int k = p.pairSteps - 1; k < nl - 1; k += p.pairSteps) sel.push_back(k);
+        if (nl > 0) sel.push_back(nl - 1);
+    }
     L = (int)sel.size();
     if (L == 0)
         throw Error(-4, "agglomeration produced no coarse level (matrix smaller than the coarsest-level size)");
@@ -1150,7 +1154,7 @@ static AmgParams toParams(const unap_hip_amg_params* p) {
     need(std::isfinite(a.chebyRatio) && a.chebyRatio > 1.0 && std::isfinite(a.chebyBoost) && a.chebyBoost > 0.0,
          "cheby_eig_ratio must exceed 1 and cheby_boost be positive");
     need(a.growth == 0 || a.growth == 1, "level_sweep_growth must be 0 or 1");
-    need(a.pairSteps >= 1 && a.pairSteps <= 3, "pair_steps must be 1, 2 or 3");
+    need(a.pairSteps >= 1 && a.pairSteps <= 8, "pair_steps must be 1 to 8");
     need(p->cycle >= -1 && p->cycle <= 1, "unknown cycle");
     a.fixedCycle = (p->cycle == UNAP_HIP_CYCLE_AUTO) ? (a.precision == 32) : (p->cycle == UNAP_HIP_CYCLE_FIXED);
     need(a.fixedCycle || a.precision == 64, "the FP32 hierarchy needs the fixed cycle");
