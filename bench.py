@@ -169,7 +169,8 @@ def main():
     x32 = x.numpy()
     ms_dev32 = S32.last_ms()
     if args.mixed_only:
-        print(json.dumps({"mixed_s": times32, "iterations": p32["nIterations"], "hierarchy": levels32}), flush=True)
+        print(json.dumps({"mixed_s": times32, "iterations": p32["nIterations"], "hierarchy": levels32,
+                          "pair_steps": args.pair_steps}), flush=True)
         return
 
     solve(S64)  # warm-up

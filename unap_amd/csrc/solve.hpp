@@ -363,6 +363,15 @@ struct Hierarchy {
     std::shared_ptr<AgglHost> ag;
     std::vector<std::unique_ptr<DevMatrix<T>>> mats;  // coarse levels (not the coarsest for FP32)
     std::vector<DBuf<int>> cs, ch, f2c;                // transfer into level l from its fine level
+    // Restriction into level l through the agglomeration levels it skips
+    // (pair steps > 2): sub-steps of two pairwise levels, each a segmented
+    // sum over its own child lists, with a vector for every intermediate level
+    struct Chain {
+        std::vector<DBuf<int>> cs, ch;
+        std::vector<DBuf<T>> buf;  // intermediate levels (one fewer than the sub-steps)
+        std::vector<int> n;        // cells of each sub-step's coarse side
+    };
+    std::vector<Chain> chain;
     std::vector<LevelWork<T>> work;                   // corr(a) src(b) t1(c) t2(d) per coarse level
     LevelWork<T> fw;                                  // finest: fCorr(a) Apsi(b) w(c) tmp(d)
     DBuf<T> fres, pre0, src32;                        // finest residual copy, spare, source cast
@@ -414,6 +423,11 @@ struct Hierarchy {
     void smooth(int si, T* x, const T* b, int nSweeps, Skip sk);
     void coarsestSolve(Skip sk);
     void vcycle(T* psi, const T* source, T* fRes, bool psiZero, Skip sk);
+    // chain[l] for level l, whose fine level is the layout fl and whose
+    // composite map starts at agglomeration level k0
+    void buildChain(int l, int k0, const Layout& fl);
+    // dst (level l) = restriction of src (its fine level)
+    void restrictInto(int l, const T* src, T* dst, Skip sk);
     // w = M^-1 r.  r32: r already in FP32 (FP32 hierarchy), or null.
     void precondition(const double* r, T* w, Skip sk, const T* r32 = nullptr);
 };

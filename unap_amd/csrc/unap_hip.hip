@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstdlib>
 #include <cmath>
 #include <cstdio>
@@ -221,6 +222,7 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
     cs.clear();
     ch.clear();
     f2c.clear();
+    chain.clear();
     work.clear();
     mats.resize(L);
     work.resize(L);
@@ -245,6 +247,7 @@ void Hierarchy<T>::build(LduObj& A, std::shared_ptr<AgglHost> agg, const AmgPara
         ch.back().upload(tr.child, c.stream);
         f2c.back().upload(tr.fineToCoarse, c.stream);
         c.sync();
+        buildChain(l, k0, fl);
         if (!(fixed && l == L - 1)) {
             const AgglDevLevel& D = ag->dev[sel[l]];
             mats[l].reset(new DevMatrix<T>);
@@ -447,11 +450,62 @@ void Hierarchy<T>::coarsestSolve(Skip sk) {
     UH_LAUNCHED();
 }
 
+// Pair steps above 2 sum ~20 children a coarse cell (4 steps), scattered over
+// several breadth-first layers of the fine level: a one-step restriction then
+// fetches about ten times its bytes.  Two pairwise levels at a time (about
+// 4.6 children each) through the agglomeration levels in between, each in its
+// own breadth-first order, the gathers stay in cache.  The sums associate
+// differently from the one-step sum (same terms).
+template <typename T>
+void Hierarchy<T>::buildChain(int l, int k0, const Layout& fl) {
+    chain.emplace_back();
+    Chain& C = chain.back();
+    const int k1 = sel[l];
+    if (k1 - k0 + 1 <= 2) return;  // one step
+    Ctx& c = Ctx::get();
+    const Layout* from = &fl;
+    for (int a = k0; a <= k1;) {
+        const int b = std::min(a + 1, k1);  // this sub-step: agglomeration levels a..b
+        const Layout& to = ag->layoutAt(order, b);
+        std::vector<int> rm = ag->levels[a].restrictMap;
+        for (int k = a + 1; k <= b; ++k)
+            for (int& v : rm) v = ag->levels[k].restrictMap[v];
+        Transfer tr = buildTransfer(*from, to, rm);
+        C.cs.emplace_back();
+        C.ch.emplace_back();
+        C.cs.back().upload(tr.childStart, c.stream);
+        C.ch.back().upload(tr.child, c.stream);
+        c.sync();
+        C.n.push_back(to.n);
+        if (b < k1) {
+            C.buf.emplace_back();
+            C.buf.back().alloc(to.n);
+        }
+        from = &to;
+        a = b + 1;
+    }
+}
+
+template <typename T>
+void Hierarchy<T>::restrictInto(int l, const T* src, T* dst, Skip sk) {
+    const Chain& C = chain[l];
+    if (C.cs.empty()) {
+        launchRowsXcd(BRestrict<T, T>{cs[l].get(), ch[l].get(), src, dst, nullptr}, nl(l), sk);
+        return;
+    }
+    const T* in = src;
+    for (size_t j = 0; j < C.cs.size(); ++j) {
+        T* out = (j + 1 < C.cs.size()) ? C.buf[j].get() : dst;
+        launchRowsXcd(BRestrict<T, T>{C.cs[j].get(), C.ch[j].get(), in, out, nullptr}, C.n[j], sk);
+        in = out;
+    }
+}
+
 template <typename T>
 void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, bool psiZero, Skip sk) {
     const int coarsest = L - 1;
     // restrict the finest residual (Vcycle.cpp:43-45)
-    launchRowsXcd(BRestrict<T, T>{cs[0].get(), ch[0].get(), fRes, work[0].b.get(), nullptr}, nl(0), sk);
+    restrictInto(0, fRes, work[0].b.get(), sk);
     for (int l = 0; l < coarsest; ++l) {
         LevelWork<T>& W = work[l];
         const int n = nl(l);
@@ -466,8 +520,7 @@ void Hierarchy<T>::vcycle(T* psi, const T* source, T* fRes, bool psiZero, Skip s
             }
             launchRows(BSubAmul<T>{mat(l).view(), W.a.get(), W.b.get()}, n, sk);
         }
-        launchRowsXcd(BRestrict<T, T>{cs[l + 1].get(), ch[l + 1].get(), W.b.get(), work[l + 1].b.get(), nullptr},
-                   nl(l + 1), sk);
+        restrictInto(l + 1, W.b.get(), work[l + 1].b.get(), sk);
     }
     coarsestSolve(sk);
     for (int l = coarsest - 1; l >= 0; --l) {
@@ -1201,10 +1254,11 @@ int unap_hip_precond_create(unap_hip_ldu A, int type, const unap_hip_amg_params*
                 rpos.resize(H.n);
                 for (int q = 0; q < H.n; ++q) rpos[q] = rin[L.bwdRows[q]];
             }
+            std::vector<float> r32, r32pos;  // host sources of the copies: alive until the sync below
             if (o.lu32) {
-                std::vector<float> r32(rin.begin(), rin.end());
+                r32.assign(rin.begin(), rin.end());
                 o.rD32.upload(r32, c.stream);
-                std::vector<float> r32pos(rpos.begin(), rpos.end());
+                r32pos.assign(rpos.begin(), rpos.end());
                 o.rD32pos.upload(r32pos, c.stream);
                 if (type == UNAP_HIP_PRECOND_DIC) A->o.dev32(o.order).buildValU(H, L, c.stream);
             } else if (type != UNAP_HIP_PRECOND_DIAG) {
@@ -1541,11 +1595,11 @@ static void profileKernel(LduObj& L, int kernel, long long reps, double* avgMs, 
     kStateInit<<<1, 1, 0, c.stream>>>(est, 0, 0, 1, 0, nullptr, 0);
     kLanczosMaxEig<<<1, 1, 0, c.stream>>>(est, 2, 1.1, 1.5);  // some finite table
     DBuf<double> rD;
-    std::vector<double> rdh;
+    std::vector<double> rdh, rin;  // host source of the copy: alive until the timing's synchronisation
     if (kernel == UNAP_HIP_KERNEL_DIC) {
         if (!lay.sweepable) throw Error(-3, "DIC needs owner-sorted faces");
         rdh = dicReciprocalD(L.host);
-        std::vector<double> rin(n);
+        rin.resize(n);
         for (int i = 0; i < n; ++i) rin[i] = rdh[lay.perm[i]];
         rD.upload(rin, c.stream);
         M64.buildValU(L.host, lay, c.stream);

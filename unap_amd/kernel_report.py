@@ -134,7 +134,7 @@ def report(trace_csv, bench_json):
     out.append(f"all kernels: {tot_b / 1e9:.1f} GB algorithmic in {busy / 1e6:.1f} ms busy = "
                f"{tot_b / (busy * 1e-9) / 1e9:.0f} GB/s")
     out.append("")
-    out += per_level(seg, lv)
+    out += per_level(seg, lv, iters, int(info.get("pair_steps", 2)))
     return "\n".join(out)
 
 
@@ -148,13 +148,37 @@ def _by_position(seg, pattern, order):
     return {lvl: sum(v) / len(v) for lvl, v in acc.items()}
 
 
-def per_level(seg, lv):
+def restrict_order(seg, L, iters, pair_steps):
+    """The level each restriction launch of one V-cycle fills: with pair
+    steps above 2 a level is reached in sub-steps of two pairwise levels
+    (ceil(pair_steps / 2) launches; the last level takes the rest), else one
+    launch a level."""
+    r = sum(1 for x in seg if "BRestrict<float, float>" in x["Kernel_Name"]) // max(1, iters)
+    per = [1] * L if pair_steps <= 2 else [(pair_steps + 1) // 2] * L
+    per[-1] = max(1, r - sum(per[:-1]))
+    return [lvl for lvl in range(L) for _ in range(per[lvl])]
+
+
+def _by_group(seg, pattern, order):
+    """Mean duration (ns) per V-cycle and level of the launches whose name
+    contains `pattern`, summed over the launches a level takes (order: the
+    level of each launch within one V-cycle)."""
+    ds = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in seg if pattern in r["Kernel_Name"]]
+    acc = collections.defaultdict(float)
+    cycles = len(ds) // len(order)
+    for i, d in enumerate(ds[:cycles * len(order)]):
+        acc[order[i % len(order)]] += d
+    return {lvl: v / cycles for lvl, v in acc.items()} if cycles else {}
+
+
+def per_level(seg, lv, iters=1, pair_steps=2):
     """Per-level means from the fixed launch order of a V-cycle: Chebyshev
     sweeps go coarse to fine (4 a level, the coarsest level unsmoothed),
-    restrictions fine to coarse, prolongations coarse to fine."""
+    restrictions fine to coarse (all the launches into a level summed),
+    prolongations coarse to fine."""
     L = len(lv) - 1
     cheb = _by_position(seg, "BCheby<float>", [lvl for lvl in range(L - 1, 0, -1) for _ in range(4)] + [0] * 4)
-    rst = _by_position(seg, "BRestrict<float, float>", list(range(L)))
+    rst = _by_group(seg, "BRestrict<float, float>", restrict_order(seg, L, iters, pair_steps))
     pro = _by_position(seg, "BProlong<float>", list(range(L - 1, -1, -1)))
     out = ["| level | cells | entries/row | Chebyshev us | Chebyshev GB/s | restrict to next us | "
            "prolong from next us |",
@@ -289,8 +313,10 @@ def traffic(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, bench_json):
     info = json.loads([ln for ln in open(bench_json) if ln.startswith("{")][-1])
     lv = _levels(info["hierarchy"])
     L = len(lv) - 1
+    rows0 = _counter_rows(fetch_csv)
+    rorder = restrict_order(rows0[solve_start(rows0):], L, int(info["iterations"]), int(info.get("pair_steps", 2)))
     orders = {"BCheby<float>": [lvl for lvl in range(L - 1, 0, -1) for _ in range(4)] + [0] * 4,
-              "BRestrict<float, float>": list(range(L)),
+              "BRestrict<float, float>": rorder,
               "BProlong<float>": list(range(L - 1, -1, -1)),
               "BAmulDot": [0]}
     alg = {"BCheby<float>": lambda l: 28 * lv[l][0] + 16 * lv[l][1],
@@ -320,8 +346,9 @@ def traffic(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, bench_json):
            "|---|---|---|---|---|---|---|---|---|"]
     for (key, lvl), (c, rd, wr) in sorted(acc.items(), key=lambda kv: (list(orders).index(kv[0][0]), kv[0][1])):
         b = alg[key](lvl)
-        t = (rd + wr) / c
-        out.append(f"| {labels[key]} | {lvl} | {lv[lvl][0]} | {c} | {rd / c / 1e6:.1f} | {wr / c / 1e6:.1f} | "
+        k = c / orders[key].count(lvl)  # calls of the level's operation (a restriction may take several launches)
+        t = (rd + wr) / k
+        out.append(f"| {labels[key]} | {lvl} | {lv[lvl][0]} | {c} | {rd / k / 1e6:.1f} | {wr / k / 1e6:.1f} | "
                    f"{t / 1e6:.1f} | {b / 1e6:.1f} | {t / b:.2f} |")
     return "\n".join(out)
 
