@@ -346,7 +346,9 @@ def traffic(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, bench_json):
            "|---|---|---|---|---|---|---|---|---|"]
     for (key, lvl), (c, rd, wr) in sorted(acc.items(), key=lambda kv: (list(orders).index(kv[0][0]), kv[0][1])):
         b = alg[key](lvl)
-        k = c / orders[key].count(lvl)  # calls of the level's operation (a restriction may take several launches)
+        # a restriction into a level may take several launches (sub-steps):
+        # its row is per restriction; the other rows are per launch
+        k = c / orders[key].count(lvl) if key.startswith("BRestrict") else c
         t = (rd + wr) / k
         out.append(f"| {labels[key]} | {lvl} | {lv[lvl][0]} | {c} | {rd / k / 1e6:.1f} | {wr / k / 1e6:.1f} | "
                    f"{t / 1e6:.1f} | {b / 1e6:.1f} | {t / b:.2f} |")
