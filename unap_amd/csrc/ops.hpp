@@ -425,6 +425,7 @@ struct BCheby {
     T* w;
     const KState* est;  // coefficient table
     int sweep;
+    int last = 0;  // the call's last sweep: w is not read again (the next call starts at sweep 0)
     __device__ void operator()(int i) const {
         const T ax = rowAmul<T, T, T, Shape>(A, i, xin);
         const T rd = (T)1 / ldL<Shape>(&A.diag[i]);
@@ -435,7 +436,7 @@ struct BCheby {
         } else {
             wi = (T)est->cheb[3 * sweep + 1] * ldL<Shape>(&w[i]) + (T)est->cheb[3 * sweep + 2] * rd * (bi - ax);
         }
-        stL<Shape>(&w[i], wi);
+        if (!last) stL<Shape>(&w[i], wi);
         stL<Shape>(&xout[i], xin[i] + wi);
     }
 };

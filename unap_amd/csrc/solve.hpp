@@ -234,9 +234,9 @@ void chebySweeps(DevMatrix<T>& M, T* x, const T* b, int nSweeps, const KState* e
     T* nxt = tmp;
     for (int s = 0; s < nSweeps; ++s) {
         if (M.shape == kShapeBatch4)
-            launchRows(BCheby<T, kShapeBatch4>{M.view(), cur, nxt, b, w, est, s}, M.n, sk);
+            launchRows(BCheby<T, kShapeBatch4>{M.view(), cur, nxt, b, w, est, s, s == nSweeps - 1}, M.n, sk);
         else  // the large levels: rows in XCD slabs, so each L2 holds the x halo of one slab
-            launchRowsXcd(BCheby<T>{M.view(), cur, nxt, b, w, est, s}, M.n, sk);
+            launchRowsXcd(BCheby<T>{M.view(), cur, nxt, b, w, est, s, s == nSweeps - 1}, M.n, sk);
         std::swap(cur, nxt);
     }
     if (cur != x) launchRows(BCopy<T, T>{cur, x}, M.n, sk);  // a kernel, not a copy node: graph-capturable
