@@ -349,6 +349,30 @@ def test_pair_steps_levels(cfg, N):
         assert [tuple(v) for v in P.amg_levels()] == want, ps
 
 
+@pytest.mark.parametrize("ps", [3, 4])
+def test_mixed_fcg_deep_pair_steps(oracle, ps):
+    # three or four pairwise steps a level: the restrictions run in sub-steps
+    # through the agglomeration levels in between; the mixed solve meets the
+    # same accuracy bar, and a repeat is bit for bit the same
+    s = _sys(4, 32)
+    A = hip.LduMatrix.from_system(s)
+    P = hip.Precond(A, "AMG", amg=hip.amg_params(smoother=2, n_cells_coarsest=50, n_vcycles=1, n_post_sweeps=4,
+                                                   n_finest_sweeps=4, precision=32, pair_steps=ps,
+                                                   cycle=hip.CYCLE_FIXED, scaling=0, cheby_eig_ratio=5.0,
+                                                   cheby_boost=1.0))
+    assert len(P.amg_levels()) >= 3
+    S = hip.Solver("FCG", P, rel_tol=1e-8, max_iter=300)
+    x, p = S.solve(A, s.b)
+    x2, p2 = S.solve(A, s.b)
+    assert p["converged"] and np.array_equal(x, x2) and np.array_equal(p["history"], p2["history"])
+    M = oracle.matrix(s)
+    xo, _ = M.solve("PCG", "DIC", s.b, rel_tol=1e-13, max_iter=5000)
+    x8, _ = M.solve("PCG", "DIC", s.b, rel_tol=1e-8, max_iter=5000)
+    assert np.linalg.norm(s.b - M.spmv(x)) <= 2e-8 * np.linalg.norm(s.b)
+    e_mixed = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    assert e_mixed <= 10 * np.linalg.norm(x8 - xo) / np.linalg.norm(xo) + 1e-12
+
+
 def test_mixed_fcg_cheby_interval(oracle):
     # the mixed cycle's Chebyshev smoothers over [lambda/5, lambda] instead of
     # the reference's [lambda/1.5, 1.1 lambda]: the same accuracy bar, and on
