@@ -292,9 +292,10 @@ def config_legs(N, with_cpu, which=("c1", "c2", "c3"), reps=3, with_mixed=True):
     # inside an FP64 flexible CG / BiCGStab
     mixed_legs = {
         "c1": ("FCG", "DIC", {"precision": 32}, "FCG + FP32 DIC"),
-        "c2": ("FCG", "AMG", {"precision": 32, "smoother": 1, "n_cells_coarsest": 500, "n_vcycles": 1,
-                              "cycle": hip.CYCLE_FIXED},
-               "FCG + FP32 AMG (Gauss-Seidel, fixed cycle, dense FP64 coarsest)"),
+        "c2": ("FCG", "AMG", {"precision": 32, "smoother": 1, "n_cells_coarsest": 4000, "n_vcycles": 1,
+                              "cycle": hip.CYCLE_FIXED, "pair_steps": 4, "level_sweep_growth": 0},
+               "FCG + FP32 AMG (Gauss-Seidel V(0,2), fixed cycle, 4 pairwise steps a level, coarsest threshold "
+               "4000, dense FP64 coarsest)"),
         "c3": ("FBiCGStab", "DILU", {"precision": 32}, "flexible BiCGStab + FP32 DILU"),
     }
     # the FP64 solution the errors are measured against
