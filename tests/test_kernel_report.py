@@ -64,3 +64,19 @@ def test_one_off_cast_counted_per_launch():
     name = "void unap_hip::kRows<unap_hip::BCopy<double, float> >(unap_hip::BCopy<double, float>, int, unap_hip::Skip)"
     assert kernel_report.alg_bytes(name, lv, 83, 1) == 12 * 1000
     assert kernel_report.alg_bytes(name, lv, 83) == 83 * 12 * 1000
+
+
+def test_restriction_sub_steps_per_level():
+    # with pair steps above 2 every level but the last is reached in
+    # ceil(pair_steps / 2) restriction launches; the last level takes the
+    # launches left over in each V-cycle
+    name = "void unap_hip::kRowsXcd<unap_hip::BRestrict<float, float> >(unap_hip::BRestrict<float, float>, int)"
+    seg = [{"Kernel_Name": name, "Start_Timestamp": "0", "End_Timestamp": "1"}] * 12
+    assert kernel_report.restrict_order(seg, 3, 2, 4) == [0, 0, 1, 1, 2, 2]
+    assert kernel_report.restrict_order(seg, 3, 2, 2) == [0, 1, 2, 2, 2, 2]
+    assert kernel_report.restrict_order(seg[:6], 3, 2, 2) == [0, 1, 2]
+    assert kernel_report.restrict_order(seg[:10], 3, 2, 4) == [0, 0, 1, 1, 2]
+    # summed per level and V-cycle
+    durs = [{"Kernel_Name": name, "Start_Timestamp": "0", "End_Timestamp": str(d)} for d in (10, 20, 3, 4, 1, 2) * 2]
+    got = kernel_report._by_group(durs, "BRestrict<float, float>", [0, 0, 1, 1, 2, 2])
+    assert got == {0: 30.0, 1: 7.0, 2: 3.0}
