@@ -50,8 +50,11 @@ def main():
     ap.add_argument("--mixed-n-coarsest", type=int, default=4000,
                     help="coarsest-level threshold of the mixed algorithm (dense FP64 inverse)")
     ap.add_argument("--pair-steps", type=int, default=4, help="pairwise steps per level of the mixed hierarchy")
-    ap.add_argument("--mixed-cheby-ratio", type=float, default=5.0,
-                    help="Chebyshev interval of the mixed cycle: lower end lambda / ratio (the reference: 1.5)")
+    ap.add_argument("--mixed-cheby-ratio", type=float, default=3.0,
+                    help="Chebyshev interval of the mixed cycle's coarse levels: lower end lambda / ratio "
+                         "(the reference: 1.5 on every level)")
+    ap.add_argument("--mixed-cheby-ratio-finest", type=float, default=12.0,
+                    help="the same on the finest level (0: --mixed-cheby-ratio)")
     ap.add_argument("--mixed-cheby-boost", type=float, default=1.0,
                     help="Chebyshev interval of the mixed cycle: upper end lambda * boost (the reference: 1.1)")
     ap.add_argument("--mixed-scaling", type=int, default=0,
@@ -119,7 +122,7 @@ def main():
     common = dict(smoother=2, n_vcycles=1, n_post_sweeps=4, n_finest_sweeps=4)
     mixed = dict(level_sweep_growth=0, pair_steps=args.pair_steps, n_cells_coarsest=args.mixed_n_coarsest,
                  cycle=hip.CYCLE_FIXED, scaling=args.mixed_scaling, cheby_eig_ratio=args.mixed_cheby_ratio,
-                 cheby_boost=args.mixed_cheby_boost, **common)
+                 cheby_eig_ratio_finest=args.mixed_cheby_ratio_finest, cheby_boost=args.mixed_cheby_boost, **common)
     t0 = time.time()
     P32 = hip.Precond(A, "AMG", amg=hip.amg_params(precision=32, **mixed))
     t_setup32 = time.time() - t0
@@ -246,8 +249,11 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"C4: {N}^3 permuted log-uniform(1e-4..1) Poisson LDU, AMG-PCG to rel.res 1e-8",
                    "n_cells": s.n, "n_faces": s.nfaces,
-                   "mixed": f"FCG(FP64) + FP32 AMG V(0,4) Chebyshev(4) over [lambda/{args.mixed_cheby_ratio:g}, "
-                            f"{args.mixed_cheby_boost:g} lambda], {args.pair_steps} pairwise steps a level, "
+                   "mixed": f"FCG(FP64) + FP32 AMG V(0,4) Chebyshev(4) over [lambda/"
+                            f"{args.mixed_cheby_ratio_finest or args.mixed_cheby_ratio:g}, "
+                            f"{args.mixed_cheby_boost:g} lambda] on the finest level and [lambda/"
+                            f"{args.mixed_cheby_ratio:g}, {args.mixed_cheby_boost:g} lambda] on the coarse ones, "
+                            f"{args.pair_steps} pairwise steps a level, "
                             f"coarsest threshold {args.mixed_n_coarsest}, dense FP64 inverse (GPU Gauss-Jordan), "
                             f"correction scaling {['none', 'coarse levels', 'reference'][args.mixed_scaling]}",
                    "fp64": "PCG + FP64 AMG V(0,4+level) Chebyshev(4) over [lambda/1.5, 1.1 lambda] (the reference's), "

@@ -168,6 +168,9 @@ typedef struct {
                                        (chebySmoother.hpp:46-47, chebySmoother.cpp:63-64);
                                        cheby_eig_ratio > 1, cheby_boost > 0 */
     double cheby_boost;
+    double cheby_eig_ratio_finest;  /* the finest level's ratio in place of cheby_eig_ratio;
+                                       0 (default): cheby_eig_ratio on every level; else > 1.
+                                       No counterpart in the reference (one ratio for all levels) */
 } unap_hip_amg_params;
 #define UNAP_HIP_CYCLE_AUTO (-1)
 #define UNAP_HIP_CYCLE_REFERENCE 0

@@ -397,6 +397,31 @@ def test_mixed_fcg_cheby_interval(oracle):
     assert its[5.0] <= its[1.5]
 
 
+def test_mixed_fcg_cheby_ratio_finest(oracle):
+    # cheby_eig_ratio_finest replaces the ratio on the finest level only: equal
+    # to cheby_eig_ratio it changes nothing (bitwise), otherwise the finest
+    # smoother changes and the solve keeps the accuracy bar
+    s = _sys(4, 24)
+    A = hip.LduMatrix.from_system(s)
+    base = dict(smoother=2, n_cells_coarsest=50, n_vcycles=1, n_post_sweeps=4, n_finest_sweeps=4, precision=32,
+                pair_steps=2, cycle=hip.CYCLE_FIXED, scaling=0, cheby_eig_ratio=3.0, cheby_boost=1.0)
+    out = {}
+    for fin in (0.0, 3.0, 12.0):
+        P = hip.Precond(A, "AMG", amg=hip.amg_params(cheby_eig_ratio_finest=fin, **base))
+        x, p = hip.Solver("FCG", P, rel_tol=1e-8, max_iter=300).solve(A, s.b)
+        assert p["converged"], fin
+        out[fin] = (x, p["nIterations"])
+    assert np.array_equal(out[0.0][0], out[3.0][0]) and out[0.0][1] == out[3.0][1]
+    assert not np.array_equal(out[0.0][0], out[12.0][0])
+    M = oracle.matrix(s)
+    xo, _ = M.solve("PCG", "DIC", s.b, rel_tol=1e-13, max_iter=5000)
+    x8, _ = M.solve("PCG", "DIC", s.b, rel_tol=1e-8, max_iter=5000)
+    x = out[12.0][0]
+    assert np.linalg.norm(s.b - M.spmv(x)) <= 2e-8 * np.linalg.norm(s.b)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 10 * np.linalg.norm(x8 - xo) / np.linalg.norm(xo) + 1e-12
+    print("FCG iterations, finest ratio 3 / 12:", out[3.0][1], out[12.0][1])
+
+
 @pytest.mark.parametrize("n", [1, 31, 64, 100, 700])
 def test_dense_inverse_spd(n):
     # blocked Gauss-Jordan with FP64 MFMA updates against numpy's inverse

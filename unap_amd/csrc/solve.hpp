@@ -349,9 +349,11 @@ struct AmgParams {
     int pairSteps = 1;  // agglomeration steps per hierarchy level
     bool fixedCycle = false;  // estimates at setup, dense FP64 coarsest solve
     int scaling = 2;          // 2 all but the two coarsest levels, 1 coarse levels only, 0 none
-    // Chebyshev interval [lambda / chebyRatio, lambda * chebyBoost] of every
+    // Chebyshev interval [lambda / ratioOf(level), lambda * chebyBoost] of every
     // level (chebySmoother.cpp:63-64; the reference's 1.5 and 1.1 by default)
     double chebyRatio = 1.5, chebyBoost = 1.1;
+    double chebyRatioFinest = 0;  // > 0: the finest level's ratio in place of chebyRatio
+    double ratioOf(int si) const { return (si == 0 && chebyRatioFinest > 0) ? chebyRatioFinest : chebyRatio; }
 };
 
 // The V-cycle of MGSolver (Vcycle.cpp:27-332) in precision T on a resident

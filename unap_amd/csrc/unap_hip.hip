@@ -380,7 +380,7 @@ void Hierarchy<T>::setupNumeric() {
                 bb.upload(bh, c.stream);
                 launchRows(BFill<T>{W.a.get(), (T)0}, M.n);
                 eigenEstimate<T>(M, W.a.get(), bb.get(), 10, est.get() + si, W.b.get(), W.c.get(),
-                                 W.d.get(), Skip{nullptr, nullptr}, prm.chebyBoost, prm.chebyRatio);
+                                 W.d.get(), Skip{nullptr, nullptr}, prm.chebyBoost, prm.ratioOf(si));
                 c.sync();
             }
             for (auto& f : firstCall) f = 0;
@@ -401,7 +401,7 @@ void Hierarchy<T>::smooth(int si, T* x, const T* b, int nSweeps, Skip sk) {
     T* v2 = (si == 0) ? W.b.get() : W.d.get();
     T* v3 = (si == 0) ? W.c.get() : W.e.get();
     if (firstCall[si] && !prm.fixedCycle) {
-        eigenEstimate<T>(M, x, b, 10, est.get() + si, v1, v2, v3, sk, prm.chebyBoost, prm.chebyRatio);
+        eigenEstimate<T>(M, x, b, 10, est.get() + si, v1, v2, v3, sk, prm.chebyBoost, prm.ratioOf(si));
         firstCall[si] = 0;
         return;
     }
@@ -1185,6 +1185,7 @@ int unap_hip_amg_params_default(unap_hip_amg_params* p) {
         p->scaling = 2;
         p->cheby_eig_ratio = 1.5;
         p->cheby_boost = 1.1;
+        p->cheby_eig_ratio_finest = 0.0;
     });
 }
 
@@ -1204,9 +1205,12 @@ static AmgParams toParams(const unap_hip_amg_params* p) {
     a.scaling = (int)p->scaling;
     a.chebyRatio = p->cheby_eig_ratio;
     a.chebyBoost = p->cheby_boost;
+    a.chebyRatioFinest = p->cheby_eig_ratio_finest;
     need(a.scaling >= 0 && a.scaling <= 2, "scaling must be 0, 1 or 2");
     need(std::isfinite(a.chebyRatio) && a.chebyRatio > 1.0 && std::isfinite(a.chebyBoost) && a.chebyBoost > 0.0,
          "cheby_eig_ratio must exceed 1 and cheby_boost be positive");
+    need(a.chebyRatioFinest == 0.0 || (std::isfinite(a.chebyRatioFinest) && a.chebyRatioFinest > 1.0),
+         "cheby_eig_ratio_finest must be 0 or exceed 1");
     need(a.growth == 0 || a.growth == 1, "level_sweep_growth must be 0 or 1");
     need(a.pairSteps >= 1 && a.pairSteps <= 8, "pair_steps must be 1 to 8");
     need(p->cycle >= -1 && p->cycle <= 1, "unknown cycle");

@@ -33,7 +33,8 @@ class AmgParams(ctypes.Structure):
                 ("n_vcycles", _i64), ("n_pre_sweeps", _i64), ("n_post_sweeps", _i64),
                 ("n_finest_sweeps", _i64), ("precision", _i64), ("level_sweep_growth", _i64),
                 ("pair_steps", _i64), ("cycle", _i64), ("scaling", _i64),
-                ("cheby_eig_ratio", ctypes.c_double), ("cheby_boost", ctypes.c_double)]
+                ("cheby_eig_ratio", ctypes.c_double), ("cheby_boost", ctypes.c_double),
+                ("cheby_eig_ratio_finest", ctypes.c_double)]
 
 
 class Perf(ctypes.Structure):
@@ -337,7 +338,7 @@ def amg_params(**kw):
     p = AmgParams()
     _check(lib().unap_hip_amg_params_default(ctypes.byref(p)))
     for k, v in kw.items():
-        setattr(p, k, float(v) if k in ("cheby_eig_ratio", "cheby_boost") else int(v))
+        setattr(p, k, float(v) if k in ("cheby_eig_ratio", "cheby_boost", "cheby_eig_ratio_finest") else int(v))
     return p
 
 
