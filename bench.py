@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--n-coarsest", type=int, default=500, help="coarsest-level threshold of the FP64 drop-in")
     ap.add_argument("--mixed-n-coarsest", type=int, default=4000,
                     help="coarsest-level threshold of the mixed algorithm (dense FP64 inverse)")
-    ap.add_argument("--pair-steps", type=int, default=4, help="pairwise steps per level of the mixed hierarchy")
+    ap.add_argument("--pair-steps", type=int, default=7, help="pairwise steps per level of the mixed hierarchy")
     ap.add_argument("--mixed-cheby-ratio", type=float, default=3.0,
                     help="Chebyshev interval of the mixed cycle's coarse levels: lower end lambda / ratio "
                          "(the reference: 1.5 on every level)")

@@ -349,9 +349,9 @@ def test_pair_steps_levels(cfg, N):
         assert [tuple(v) for v in P.amg_levels()] == want, ps
 
 
-@pytest.mark.parametrize("ps", [3, 4])
+@pytest.mark.parametrize("ps", [3, 4, 7])
 def test_mixed_fcg_deep_pair_steps(oracle, ps):
-    # three or four pairwise steps a level: the restrictions run in sub-steps
+    # three, four or seven pairwise steps a level: the restrictions run in sub-steps
     # through the agglomeration levels in between; the mixed solve meets the
     # same accuracy bar, and a repeat is bit for bit the same
     s = _sys(4, 32)
@@ -360,7 +360,7 @@ def test_mixed_fcg_deep_pair_steps(oracle, ps):
                                                    n_finest_sweeps=4, precision=32, pair_steps=ps,
                                                    cycle=hip.CYCLE_FIXED, scaling=0, cheby_eig_ratio=5.0,
                                                    cheby_boost=1.0))
-    assert len(P.amg_levels()) >= 3
+    assert len(P.amg_levels()) >= (3 if ps < 7 else 2)
     S = hip.Solver("FCG", P, rel_tol=1e-8, max_iter=300)
     x, p = S.solve(A, s.b)
     x2, p2 = S.solve(A, s.b)
